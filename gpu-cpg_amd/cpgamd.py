@@ -1,0 +1,277 @@
+"""ctypes binding of libcpgamd.so (include/cpg.h) and a CpGen-shaped wrapper.
+
+The wrapper mirrors gpucpg::CpGen (gpucpg/gpucpg.cuh:119-431): read_input,
+reset, report_paths, get_slacks, get_pfxt_nodes, num_verts, num_edges,
+graph_diameter and the static-cache controls.  Every call goes through the
+C ABI into the HIP engine; there is no Python or CPU fallback, and loading
+fails with an exception when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpgamd.so")
+
+
+class CpgNode(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_int), ("from_", ctypes.c_int), ("to", ctypes.c_int),
+                ("parent", ctypes.c_int), ("num_children", ctypes.c_int),
+                ("cost", ctypes.c_float)]
+
+
+class QueryOpts(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_int), ("init_split_perc", ctypes.c_float),
+                ("root_tie_quirk", ctypes.c_int), ("batch", ctypes.c_int64),
+                ("max_nodes", ctypes.c_int64), ("mem_budget_bytes", ctypes.c_int64),
+                ("endpoint_mask", ctypes.c_void_p), ("collect_stats", ctypes.c_int)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("sfxt_ms", ctypes.c_double), ("pfxt_ms", ctypes.c_double),
+                ("select_ms", ctypes.c_double), ("query_ms", ctypes.c_double),
+                ("rounds", ctypes.c_int64), ("expanded", ctypes.c_int64),
+                ("generated", ctypes.c_int64), ("hops", ctypes.c_int64),
+                ("dev_visits", ctypes.c_int64), ("roots", ctypes.c_int64),
+                ("roots_dropped", ctypes.c_int64), ("retries", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class EdgeList(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("m", ctypes.c_int64),
+                ("u", ctypes.POINTER(ctypes.c_int)), ("v", ctypes.POINTER(ctypes.c_int)),
+                ("w", ctypes.POINTER(ctypes.c_float))]
+
+
+# name -> (restype, argtypes); the list is the C ABI declared in include/cpg.h
+_H = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_PI = ctypes.POINTER(ctypes.c_int)
+_PF = ctypes.POINTER(ctypes.c_float)
+ABI = {
+    "cpg_create": (_H, []),
+    "cpg_destroy": (None, [_H]),
+    "cpg_last_error": (ctypes.c_char_p, []),
+    "cpg_load_text": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
+    "cpg_load_edges": (ctypes.c_int, [_H, ctypes.c_int, _I64, _PI, _PI, _PF]),
+    "cpg_load_csr": (ctypes.c_int, [_H, ctypes.c_int, _I64, _PI, _PI, _PF]),
+    "cpg_get_csr": (ctypes.c_int, [_H, _PI, _PI, _PF]),
+    "cpg_num_verts": (_I64, [_H]),
+    "cpg_num_edges": (_I64, [_H]),
+    "cpg_graph_diameter": (ctypes.c_int, [_H]),
+    "cpg_reset": (ctypes.c_int, [_H]),
+    "cpg_query_opts_default": (None, [ctypes.POINTER(QueryOpts)]),
+    "cpg_query": (ctypes.c_int, [_H, _I64, ctypes.POINTER(QueryOpts)]),
+    "cpg_num_paths": (_I64, [_H]),
+    "cpg_get_costs": (_I64, [_H, _I64, _PF]),
+    "cpg_get_nodes": (_I64, [_H, _I64, ctypes.POINTER(CpgNode)]),
+    "cpg_get_path_vertices": (_I64, [_H, _I64, _PI, _I64]),
+    "cpg_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
+    "cpg_sfxt": (ctypes.c_int, [_H]),
+    "cpg_get_sfxt": (ctypes.c_int, [_H, _PI, _PI, _PI]),
+    "cpg_cache_enable": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cpg_cache_clear": (ctypes.c_int, [_H]),
+    "cpg_cache_hits": (ctypes.c_int, [_H]),
+    "cpg_cache_misses": (ctypes.c_int, [_H]),
+    "cpg_gen_preset": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(EdgeList)]),
+    "cpg_edge_list_free": (None, [ctypes.POINTER(EdgeList)]),
+    "cpg_preset_k": (_I64, [ctypes.c_int, ctypes.c_int]),
+    "cpg_write_text": (ctypes.c_int, [ctypes.POINTER(EdgeList), ctypes.c_char_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libcpgamd.so once; raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class CpgError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc is None or (isinstance(rc, int) and rc < 0):
+        raise CpgError(lib().cpg_last_error().decode())
+    return rc
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_PI)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_PF)
+
+
+def gen_preset(which, index=0, unit_wgts=False):
+    """Seeded synthetic workload as (n, u, v, w) numpy arrays (host only)."""
+    el = EdgeList()
+    _check(lib().cpg_gen_preset(which, index, 1 if unit_wgts else 0, ctypes.byref(el)))
+    try:
+        m = el.m
+        u = np.ctypeslib.as_array(el.u, shape=(m,)).copy() if m else np.zeros(0, np.int32)
+        v = np.ctypeslib.as_array(el.v, shape=(m,)).copy() if m else np.zeros(0, np.int32)
+        w = np.ctypeslib.as_array(el.w, shape=(m,)).copy() if m else np.zeros(0, np.float32)
+        return el.n, u, v, w
+    finally:
+        lib().cpg_edge_list_free(ctypes.byref(el))
+
+
+def preset_k(which, index=0):
+    return _check(lib().cpg_preset_k(which, index))
+
+
+def write_text(n, u, v, w, path):
+    u = np.ascontiguousarray(u, np.int32)
+    v = np.ascontiguousarray(v, np.int32)
+    w = np.ascontiguousarray(w, np.float32)
+    el = EdgeList(n, len(u), _iptr(u), _iptr(v), _fptr(w))
+    _check(lib().cpg_write_text(ctypes.byref(el), path.encode()))
+
+
+class CpGen:
+    """gpucpg::CpGen-shaped handle over the C ABI."""
+
+    def __init__(self):
+        self.h = lib().cpg_create()
+        if not self.h:
+            raise CpgError(lib().cpg_last_error().decode())
+        self._k = 0
+
+    def close(self):
+        if self.h:
+            lib().cpg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- input (gpucpg/gpucpg.cu:453-548)
+    def read_input(self, filename, ignore_wgts=False):
+        _check(lib().cpg_load_text(self.h, filename.encode(), 1 if ignore_wgts else 0))
+
+    def load_edges(self, n, u, v, w):
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        w = np.ascontiguousarray(w, np.float32)
+        _check(lib().cpg_load_edges(self.h, int(n), len(u), _iptr(u), _iptr(v), _fptr(w)))
+
+    def load_csr(self, adjp, adjncy, wgts):
+        adjp = np.ascontiguousarray(adjp, np.int32)
+        adjncy = np.ascontiguousarray(adjncy, np.int32)
+        wgts = np.ascontiguousarray(wgts, np.float32)
+        _check(lib().cpg_load_csr(self.h, len(adjp) - 1, len(adjncy), _iptr(adjp),
+                                  _iptr(adjncy), _fptr(wgts)))
+
+    def csr(self):
+        n, m = self.num_verts(), self.num_edges()
+        adjp = np.zeros(n + 1, np.int32)
+        adj = np.zeros(max(m, 1), np.int32)
+        w = np.zeros(max(m, 1), np.float32)
+        _check(lib().cpg_get_csr(self.h, _iptr(adjp), _iptr(adj), _fptr(w)))
+        return adjp, adj[:m], w[:m]
+
+    def num_verts(self):
+        return lib().cpg_num_verts(self.h)
+
+    def num_edges(self):
+        return lib().cpg_num_edges(self.h)
+
+    @property
+    def graph_diameter(self):
+        return lib().cpg_graph_diameter(self.h)
+
+    # -- query (gpucpg/gpucpg.cuh:211-250, gpucpg/gpucpg.cu:16910-21366)
+    def reset(self):
+        _check(lib().cpg_reset(self.h))
+
+    def report_paths(self, k, *args, root_tie_quirk=True, batch=0, max_nodes=0,
+                     collect_stats=False, init_split_perc=0.005, **kwargs):
+        """Runs the GPG query for the k lowest-cost paths.  Positional
+        arguments after k (max_dev_lvls, enable_compress, methods, ...) are
+        accepted for call compatibility with the reference; the engine always
+        runs its exact GPU path."""
+        o = QueryOpts()
+        lib().cpg_query_opts_default(ctypes.byref(o))
+        o.root_tie_quirk = 1 if root_tie_quirk else 0
+        o.batch = int(batch)
+        o.max_nodes = int(max_nodes)
+        o.collect_stats = 1 if collect_stats else 0
+        o.init_split_perc = float(init_split_perc)
+        _check(lib().cpg_query(self.h, int(k), ctypes.byref(o)))
+        self._k = int(k)
+
+    def num_paths(self):
+        return lib().cpg_num_paths(self.h)
+
+    def get_slacks(self, k):
+        out = np.zeros(max(int(k), 1), np.float32)
+        n = _check(lib().cpg_get_costs(self.h, int(k), _fptr(out)))
+        return out[:n]
+
+    def get_pfxt_nodes(self, k):
+        k = max(int(k), 0)
+        buf = (CpgNode * max(k, 1))()
+        n = _check(lib().cpg_get_nodes(self.h, k, buf))
+        arr = np.ctypeslib.as_array(buf)[:n]
+        return np.array(arr, copy=True)
+
+    def path_vertices(self, i):
+        cap = 1 << 16
+        while True:
+            out = np.zeros(cap, np.int32)
+            n = lib().cpg_get_path_vertices(self.h, int(i), _iptr(out), cap)
+            if n >= 0:
+                return out[:n]
+            if n <= -1000:
+                cap = -(n + 1000) + 1
+                continue
+            raise CpgError(lib().cpg_last_error().decode())
+
+    def timings(self):
+        t = Timing()
+        _check(lib().cpg_timings(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def sfxt(self):
+        n = self.num_verts()
+        dist = np.zeros(max(n, 1), np.int32)
+        succ = np.zeros(max(n, 1), np.int32)
+        level = np.zeros(max(n, 1), np.int32)
+        _check(lib().cpg_get_sfxt(self.h, _iptr(dist), _iptr(succ), _iptr(level)))
+        return dist[:n], succ[:n], level[:n]
+
+    def run_sfxt(self):
+        _check(lib().cpg_sfxt(self.h))
+
+    # -- static cache (gpucpg/gpucpg.cuh:186-189)
+    def enable_tc_pfxt_static_cache(self, enabled):
+        _check(lib().cpg_cache_enable(self.h, 1 if enabled else 0))
+
+    def clear_tc_pfxt_static_cache(self):
+        _check(lib().cpg_cache_clear(self.h))
+
+    def tc_pfxt_static_cache_hits(self):
+        return lib().cpg_cache_hits(self.h)
+
+    def tc_pfxt_static_cache_misses(self):
+        return lib().cpg_cache_misses(self.h)

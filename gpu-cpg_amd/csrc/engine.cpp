@@ -1,0 +1,674 @@
+// Host orchestration of the CPG query and the C ABI (include/cpg.h).
+//
+// The handle mirrors gpucpg::CpGen (gpucpg/gpucpg.cuh:119-431): it owns the
+// host CSR, keeps the graph and its SFXT resident on the device between
+// queries, and runs one query at a time on its own HIP stream.
+#include "../../include/cpg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "graph.hpp"
+#include "kernels.hpp"
+
+namespace cpg {
+void launch_gather_int(const int* src, const int* idx, int n, int* out, hipStream_t s);
+}
+
+using namespace cpg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void reserve(size_t want) {
+    if (want <= n && p) return;
+    release();
+    CPG_HIP(hipMalloc(&p, sizeof(T) * (want ? want : 1)));
+    n = want;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+template <class T>
+void h2d(T* d, const T* h, size_t n, hipStream_t s) {
+  if (n) CPG_HIP(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+}
+template <class T>
+void d2h(T* h, const T* d, size_t n, hipStream_t s) {
+  if (n) CPG_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+}
+
+}  // namespace
+
+struct cpg_handle {
+  HostGraph g;
+  bool loaded = false;
+  hipStream_t stream = nullptr;
+  int grid = 0;  // grid-stride kernels
+
+  // device graph
+  DevBuf<int> oadjp, oadj, iadjp, iadj, srcs;
+  DevBuf<float> owgt;
+  bool graph_on_device = false;
+
+  // SFXT
+  DevBuf<int> deps, queue, lvlp, level, dist, succ, src_dist;
+  DevBuf<float> distf;
+  DevBuf<int4> vrec, devs;
+  DevBuf<int2> devinfo;
+  DevBuf<SfxtCtl> sctl;
+  bool sfxt_valid = false;
+  int diameter = 0;
+  float max_abs_distf = 0.0f;
+  bool cache_enabled = false;
+  int cache_hits = 0, cache_misses = 0;
+  std::vector<int> h_succ;  // lazily copied for path reconstruction
+
+  // PFXT
+  DevBuf<float> n_cost;
+  DevBuf<int> n_to, n_from, n_parent, n_level;
+  DevBuf<uint32_t> live_key[2], live_idx[2], batch;
+  DevBuf<unsigned> hist_all, hist_pend;
+  DevBuf<PfxtCtl> pctl;
+  DevBuf<uint8_t> keep;
+  uint64_t node_cap = 0;
+
+  // final selection
+  DevBuf<uint64_t> skey[2];
+  DevBuf<uint32_t> sval[2];
+  DevBuf<unsigned> ghist;
+  DevBuf<unsigned long long> zc;
+  DevBuf<float> r_cost;
+  DevBuf<int> r_level, r_from, r_to, r_pidx, r_parent, rank_of;
+  int64_t nres = 0;
+  bool have_result = false;
+  std::vector<float> h_cost;
+  std::vector<int> h_level, h_from, h_to, h_parent;
+  bool host_result = false;
+
+  cpg_timing timing{};
+
+  cpg_handle() {}
+  ~cpg_handle() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void ensure_stream() {
+    if (stream) return;
+    CPG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    int dev = 0, cus = 0;
+    CPG_HIP(hipGetDevice(&dev));
+    CPG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = std::max(1, cus) * 8;
+  }
+
+  void set_graph(const EdgeList& el) {
+    HostGraph ng;
+    build_csr(el, ng);
+    g = std::move(ng);
+    loaded = true;
+    graph_on_device = false;
+    invalidate_cache_payload();
+    cache_hits = cache_misses = 0;  // read_input clears the static cache (gpucpg.cu:454)
+    reset();
+  }
+
+  void invalidate_cache_payload() {
+    sfxt_valid = false;
+    h_succ.clear();
+  }
+
+  void reset() {  // CpGen::reset, gpucpg/gpucpg.cuh:211-250
+    have_result = false;
+    host_result = false;
+    nres = 0;
+    timing = cpg_timing{};
+    if (!cache_enabled) invalidate_cache_payload();
+  }
+
+  void upload_graph() {
+    if (graph_on_device) return;
+    ensure_stream();
+    const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
+    oadjp.reserve(n + 1); oadj.reserve(m); owgt.reserve(m);
+    iadjp.reserve(n + 1); iadj.reserve(m); srcs.reserve(g.srcs.size());
+    h2d(oadjp.p, g.oadjp.data(), n + 1, stream);
+    h2d(oadj.p, g.oadj.data(), m, stream);
+    h2d(owgt.p, g.owgt.data(), m, stream);
+    h2d(iadjp.p, g.iadjp.data(), n + 1, stream);
+    h2d(iadj.p, g.iadj.data(), m, stream);
+    h2d(srcs.p, g.srcs.data(), g.srcs.size(), stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    graph_on_device = true;
+    sfxt_valid = false;
+  }
+
+  // Runs the SFXT unless a cached one is valid.  Returns elapsed ms.
+  double run_sfxt(bool count_cache) {
+    upload_graph();
+    if (sfxt_valid && cache_enabled) {
+      if (count_cache) ++cache_hits;
+      return 0.0;
+    }
+    if (cache_enabled && count_cache) ++cache_misses;
+    const double t0 = now_ms();
+    const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
+    deps.reserve(n); queue.reserve(n); lvlp.reserve(n + 2); level.reserve(n);
+    dist.reserve(n); succ.reserve(n); distf.reserve(n); vrec.reserve(n);
+    devinfo.reserve(n); devs.reserve(m); sctl.reserve(1);
+    CPG_HIP(hipMemsetAsync(sctl.p, 0, sizeof(SfxtCtl), stream));
+    if (n > 0) {
+      SfxtArgs a{};
+      a.n = g.n;
+      a.oadjp = oadjp.p; a.oadj = oadj.p; a.owgt = owgt.p;
+      a.iadjp = iadjp.p; a.iadj = iadj.p;
+      a.deps = deps.p; a.queue = queue.p; a.lvlp = lvlp.p; a.level = level.p;
+      a.dist = dist.p; a.succ = succ.p; a.distf = distf.p;
+      a.vrec = vrec.p; a.devinfo = devinfo.p; a.devs = devs.p; a.ctl = sctl.p;
+      launch_sfxt(a, stream, nullptr);
+    }
+    SfxtCtl c{};
+    d2h(&c, sctl.p, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
+    diameter = c.num_levels;
+    uint32_t bits = c.max_abs_distf_bits;
+    std::memcpy(&max_abs_distf, &bits, 4);
+    sfxt_valid = true;
+    h_succ.clear();
+    return now_ms() - t0;
+  }
+
+  void ensure_pfxt_buffers(uint64_t cap) {
+    if (cap > 0x7fffffffull) throw std::runtime_error("node capacity exceeds 2^31");
+    n_cost.reserve(cap); n_to.reserve(cap); n_from.reserve(cap);
+    n_parent.reserve(cap); n_level.reserve(cap);
+    for (int i = 0; i < 2; ++i) { live_key[i].reserve(cap); live_idx[i].reserve(cap); }
+    batch.reserve(cap);
+    hist_all.reserve(kHistBins);
+    hist_pend.reserve(kHistBins);
+    pctl.reserve(1);
+    node_cap = cap;
+  }
+
+  void query(int64_t k, const cpg_query_opts& o) {
+    if (!loaded) throw std::runtime_error("no graph loaded");
+    have_result = false;
+    host_result = false;
+    nres = 0;
+    timing = cpg_timing{};
+    const double tq0 = now_ms();
+    timing.sfxt_ms = run_sfxt(true);
+    if (k <= 0 || g.n == 0) {
+      timing.query_ms = now_ms() - tq0;
+      have_result = true;
+      return;
+    }
+    const double tp0 = now_ms();
+    // roots: sources with finite dist (gpucpg/gpucpg.cu:18502-18513)
+    const int nsrc = static_cast<int>(g.srcs.size());
+    std::vector<int> sd(nsrc);
+    src_dist.reserve(std::max(1, nsrc));
+    launch_gather_int(dist.p, srcs.p, nsrc, src_dist.p, stream);
+    d2h(sd.data(), src_dist.p, nsrc, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    std::vector<uint8_t> kp(std::max(1, nsrc), 0);
+    std::vector<float> rc;
+    rc.reserve(nsrc);
+    for (int i = 0; i < nsrc; ++i)
+      if (sd[i] != kIntMax) {
+        kp[i] = 1;
+        rc.push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+      }
+    const int64_t nroots = static_cast<int64_t>(rc.size());
+    int64_t dropped = 0;
+    if (o.root_tie_quirk && nroots > 0) {
+      // initial split, gpucpg/gpucpg.cu:19136-19180: the short pile keeps
+      // the first (int)(p*n+1) roots (ascending id) with cost <= split
+      std::vector<float> sorted = rc;
+      std::sort(sorted.begin(), sorted.end());
+      const float pn = o.init_split_perc * static_cast<float>(nroots);
+      const float split = sorted[static_cast<size_t>(pn)];
+      const int num_short = static_cast<int>(pn + 1);
+      int kept_le = 0;
+      for (int i = 0; i < nsrc; ++i) {
+        if (!kp[i]) continue;
+        const float c = static_cast<float>(sd[i]) / static_cast<float>(kScaleUp);
+        if (c > split) continue;
+        if (kept_le < num_short) ++kept_le;
+        else { kp[i] = 0; ++dropped; }
+      }
+    }
+    float minc = __builtin_huge_valf();
+    for (int i = 0; i < nsrc; ++i)
+      if (kp[i]) minc = std::min(minc, static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+    timing.roots = nroots - dropped;
+    timing.roots_dropped = dropped;
+    keep.reserve(std::max(1, nsrc));
+    h2d(keep.p, kp.data(), nsrc, stream);
+
+    const uint64_t batch_target = o.batch > 0 ? static_cast<uint64_t>(o.batch) : 131072ull;
+    uint64_t cap = o.max_nodes > 0 ? static_cast<uint64_t>(o.max_nodes)
+                                   : std::max<uint64_t>(uint64_t(1) << 22,
+                                                        8ull * static_cast<uint64_t>(k) +
+                                                            static_cast<uint64_t>(nroots) + (1u << 20));
+    cap = std::min<uint64_t>(cap, 0x7fffffffull);
+    int cur = 0;
+    PfxtCtl c{};
+    for (int attempt = 0;; ++attempt) {
+      ensure_pfxt_buffers(cap);
+      PfxtCtl init{};
+      init.ukey = kKeyInf;
+      init.klo = nroots - dropped > 0 ? float_key(minc) : 0u;
+      init.shift = window_shift(init.klo, kKeyInf);
+      init.node_cap = cap;
+      h2d(pctl.p, &init, 1, stream);
+      CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * kHistBins, stream));
+      CPG_HIP(hipMemsetAsync(hist_pend.p, 0, sizeof(unsigned) * kHistBins, stream));
+      NodeStore ns{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, cap};
+      launch_roots(srcs.p, keep.p, nsrc, dist.p, distf.p, ns, live_key[0].p, live_idx[0].p, pctl.p,
+                   stream);
+      cur = 0;
+      LiveArgs la{live_key[0].p, live_idx[0].p, nullptr, nullptr, batch.p, hist_all.p, hist_pend.p,
+                  pctl.p};
+      launch_hist_live(la, grid, stream);
+      WalkArgs wa{};
+      wa.vrec = vrec.p; wa.devinfo = devinfo.p; wa.devs = devs.p; wa.batch = batch.p;
+      wa.nodes = ns; wa.hist = hist_all.p; wa.ctl = pctl.p;
+      wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = g.max_abs_wgt;
+      wa.collect_stats = o.collect_stats;
+      bool overflow = false;
+      for (int r = 0;; ++r) {
+        if (r > 1000000) throw std::runtime_error("PFXT round limit exceeded");
+        LiveArgs l{live_key[cur].p, live_idx[cur].p, live_key[cur ^ 1].p, live_idx[cur ^ 1].p,
+                   batch.p, hist_all.p, hist_pend.p, pctl.p};
+        launch_round_begin(pctl.p, stream);
+        launch_select_batch(pctl.p, hist_pend.p, r == 0 ? ~0ull : batch_target, stream);
+        launch_make_batch(l, grid, stream);
+        wa.live_key = live_key[cur].p;
+        wa.live_idx = live_idx[cur].p;
+        launch_walk(wa, false, grid, stream);
+        launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+        launch_walk(wa, true, grid, stream);
+        launch_compact_live(l, grid, stream);
+        d2h(&c, pctl.p, 1, stream);
+        CPG_HIP(hipStreamSynchronize(stream));
+        if (c.overflow) { overflow = true; break; }
+        if (c.done) break;  // the round did nothing: live stays in `cur`
+        cur ^= 1;
+      }
+      if (!overflow) break;
+      if (attempt >= 4 || cap >= 0x7fffffffull) throw std::runtime_error("prefix-tree node capacity exceeded");
+      cap = std::min<uint64_t>(cap * 2, 0x7fffffffull);
+      timing.retries++;
+    }
+    timing.rounds = c.rounds;
+    timing.expanded = static_cast<int64_t>(c.expanded);
+    timing.generated = static_cast<int64_t>(c.node_n);
+    timing.hops = static_cast<int64_t>(c.hops);
+    timing.dev_visits = static_cast<int64_t>(c.devvis);
+    const double ts0 = now_ms();
+    timing.pfxt_ms = ts0 - tp0;
+
+    // final selection over the live set (all paths with cost <= bound)
+    const uint64_t nl = c.live_n;
+    skey[0].reserve(nl); skey[1].reserve(nl); sval[0].reserve(nl); sval[1].reserve(nl);
+    ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
+    zc.reserve(2);
+    CPG_HIP(hipMemsetAsync(zc.p, 0, 2 * sizeof(unsigned long long), stream));
+    launch_build_sort_keys(live_key[cur].p, live_idx[cur].p, nl, n_level.p, skey[0].p, sval[0].p,
+                           zc.p, stream);
+    SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
+    const int bytes[6] = {0, 1, 4, 5, 6, 7};
+    const int res = radix_sort_pairs(sb, nl, bytes, 6, stream);
+    unsigned long long z[2] = {0, 0};
+    d2h(z, zc.p, 2, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    const uint64_t avail = nl - z[1];
+    const uint64_t nout = std::min<uint64_t>(static_cast<uint64_t>(k), avail);
+    r_cost.reserve(std::max<uint64_t>(1, nout));
+    r_level.reserve(std::max<uint64_t>(1, nout)); r_from.reserve(std::max<uint64_t>(1, nout));
+    r_to.reserve(std::max<uint64_t>(1, nout)); r_pidx.reserve(std::max<uint64_t>(1, nout));
+    r_parent.reserve(std::max<uint64_t>(1, nout));
+    rank_of.reserve(std::max<uint64_t>(1, c.node_n));
+    launch_fill_int(rank_of.p, -1, c.node_n, stream);
+    NodeStore ns{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, node_cap};
+    launch_emit_report(skey[res].p, sval[res].p, nout, z[0], z[1], ns, r_cost.p, r_level.p,
+                       r_from.p, r_to.p, r_pidx.p, rank_of.p, stream);
+    launch_link_parents(r_pidx.p, rank_of.p, nout, r_parent.p, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    const double te = now_ms();
+    timing.select_ms = te - ts0;
+    timing.query_ms = te - tq0;
+    nres = static_cast<int64_t>(nout);
+    have_result = true;
+  }
+
+  void fetch_result() {
+    if (host_result) return;
+    if (!have_result) throw std::runtime_error("no query result");
+    const size_t n = static_cast<size_t>(nres);
+    h_cost.resize(n); h_level.resize(n); h_from.resize(n); h_to.resize(n); h_parent.resize(n);
+    if (n) {
+      d2h(h_cost.data(), r_cost.p, n, stream);
+      d2h(h_level.data(), r_level.p, n, stream);
+      d2h(h_from.data(), r_from.p, n, stream);
+      d2h(h_to.data(), r_to.p, n, stream);
+      d2h(h_parent.data(), r_parent.p, n, stream);
+      CPG_HIP(hipStreamSynchronize(stream));
+    }
+    host_result = true;
+  }
+
+  void fetch_succ() {
+    if (!h_succ.empty() || g.n == 0) return;
+    if (!sfxt_valid) run_sfxt(false);
+    h_succ.resize(g.n);
+    d2h(h_succ.data(), succ.p, g.n, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+  }
+};
+
+// ------------------------------------------------------------------ ABI
+
+// Runs f, turning any exception into fail_value and a thread-local message.
+template <class R, class F>
+R guarded(R fail_value, F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return fail_value;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return fail_value;
+  }
+}
+
+#define CPG_CHECK_HANDLE(h, fail_value)                 \
+  if (!(h)) {                                           \
+    g_last_error = "null handle";                       \
+    return fail_value;                                  \
+  }
+
+extern "C" {
+
+cpg_handle* cpg_create(void) {
+  return guarded<cpg_handle*>(nullptr, [&]() { return new cpg_handle(); });
+}
+
+void cpg_destroy(cpg_handle* h) { delete h; }
+
+const char* cpg_last_error(void) { return g_last_error.c_str(); }
+
+int cpg_load_text(cpg_handle* h, const char* path, int ignore_wgts) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    EdgeList el;
+    parse_text(path ? path : "", ignore_wgts != 0, el);
+    h->set_graph(el);
+    return 0;
+  });
+}
+
+int cpg_load_edges(cpg_handle* h, int n, int64_t m, const int* eu, const int* ev, const float* ew) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (n < 0 || m < 0 || (m > 0 && (!eu || !ev || !ew))) throw std::runtime_error("bad edge list");
+    EdgeList el;
+    el.n = n;
+    el.u.assign(eu, eu + m);
+    el.v.assign(ev, ev + m);
+    el.w.assign(ew, ew + m);
+    h->set_graph(el);
+    return 0;
+  });
+}
+
+int cpg_load_csr(cpg_handle* h, int n, int64_t m, const int* adjp, const int* adjncy, const float* wgts) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (n < 0 || m < 0 || !adjp || (m > 0 && (!adjncy || !wgts))) throw std::runtime_error("bad CSR");
+    if (adjp[0] != 0 || adjp[n] != m) throw std::runtime_error("CSR offsets do not span the edges");
+    EdgeList el;
+    el.n = n;
+    el.u.resize(m);
+    el.v.assign(adjncy, adjncy + m);
+    el.w.assign(wgts, wgts + m);
+    for (int u = 0; u < n; ++u) {
+      if (adjp[u + 1] < adjp[u]) throw std::runtime_error("CSR offsets decrease");
+      for (int e = adjp[u]; e < adjp[u + 1]; ++e) el.u[e] = u;
+    }
+    h->set_graph(el);
+    return 0;
+  });
+}
+
+int cpg_get_csr(cpg_handle* h, int* adjp, int* adjncy, float* wgts) {
+  CPG_CHECK_HANDLE(h, -1)
+  if (adjp) std::memcpy(adjp, h->g.oadjp.data(), sizeof(int) * h->g.oadjp.size());
+  if (adjncy) std::memcpy(adjncy, h->g.oadj.data(), sizeof(int) * h->g.oadj.size());
+  if (wgts) std::memcpy(wgts, h->g.owgt.data(), sizeof(float) * h->g.owgt.size());
+  return 0;
+}
+
+int64_t cpg_num_verts(cpg_handle* h) { return h ? h->g.n : -1; }
+int64_t cpg_num_edges(cpg_handle* h) { return h ? h->g.m : -1; }
+int cpg_graph_diameter(cpg_handle* h) { return h ? h->diameter : -1; }
+
+int cpg_reset(cpg_handle* h) {
+  CPG_CHECK_HANDLE(h, -1)
+  h->reset();
+  return 0;
+}
+
+void cpg_query_opts_default(cpg_query_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->struct_size = sizeof(cpg_query_opts);
+  o->init_split_perc = 0.005f;
+  o->root_tie_quirk = 1;
+}
+
+int cpg_query(cpg_handle* h, int64_t k, const cpg_query_opts* o) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    cpg_query_opts d;
+    cpg_query_opts_default(&d);
+    if (o) {
+      if (o->struct_size != static_cast<int>(sizeof(cpg_query_opts)))
+        throw std::runtime_error("cpg_query_opts.struct_size mismatch");
+      d = *o;
+    }
+    if (d.endpoint_mask) throw std::runtime_error("endpoint_mask is not supported by this build");
+    h->query(k, d);
+    return 0;
+  });
+}
+
+int64_t cpg_num_paths(cpg_handle* h) { return h && h->have_result ? h->nres : -1; }
+
+int64_t cpg_get_costs(cpg_handle* h, int64_t k, float* out) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int64_t>(-1, [&]() {
+    h->fetch_result();
+    const int64_t n = std::min<int64_t>(std::max<int64_t>(k, 0), h->nres);
+    if (n && out) std::memcpy(out, h->h_cost.data(), sizeof(float) * n);
+    return n;
+  });
+}
+
+int64_t cpg_get_nodes(cpg_handle* h, int64_t k, cpg_node* out) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int64_t>(-1, [&]() {
+    h->fetch_result();
+    const int64_t n = std::min<int64_t>(std::max<int64_t>(k, 0), h->nres);
+    for (int64_t i = 0; i < n && out; ++i) {
+      out[i].level = h->h_level[i];
+      out[i].from = h->h_from[i];
+      out[i].to = h->h_to[i];
+      out[i].parent = h->h_parent[i];
+      out[i].num_children = 0;
+      out[i].cost = h->h_cost[i];
+    }
+    return n;
+  });
+}
+
+int64_t cpg_get_path_vertices(cpg_handle* h, int64_t i, int* out, int64_t cap) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int64_t>(-1, [&]() {
+    h->fetch_result();
+    if (i < 0 || i >= h->nres) throw std::runtime_error("path index out of range");
+    h->fetch_succ();
+    // deviations from the leaf up to the root
+    std::vector<std::pair<int, int>> devs;  // (from, to)
+    int64_t r = i;
+    int source = -1;
+    for (int guard = 0; r >= 0; ++guard) {
+      if (guard > (1 << 26)) throw std::runtime_error("parent chain does not terminate");
+      if (h->h_from[r] < 0) { source = h->h_to[r]; break; }
+      devs.emplace_back(h->h_from[r], h->h_to[r]);
+      const int p = h->h_parent[r];
+      if (p < 0) throw std::runtime_error("parent path not among the reported paths");
+      r = p;
+    }
+    std::reverse(devs.begin(), devs.end());
+    std::vector<int> path;
+    int v = source;
+    size_t di = 0;
+    while (v != -1) {
+      path.push_back(v);
+      if (di < devs.size() && devs[di].first == v) {
+        v = devs[di].second;
+        ++di;
+      } else {
+        v = h->h_succ[v];
+      }
+      if (path.size() > static_cast<size_t>(h->g.n) + 1) throw std::runtime_error("path does not end");
+    }
+    if (di != devs.size()) throw std::runtime_error("deviation vertex not on the suffix chain");
+    const int64_t len = static_cast<int64_t>(path.size());
+    if (len > cap) return -len - 1000;
+    if (out) std::memcpy(out, path.data(), sizeof(int) * len);
+    return len;
+  });
+}
+
+int cpg_timings(cpg_handle* h, cpg_timing* out) {
+  CPG_CHECK_HANDLE(h, -1)
+  if (out) *out = h->timing;
+  return 0;
+}
+
+int cpg_sfxt(cpg_handle* h) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (!h->loaded) throw std::runtime_error("no graph loaded");
+    h->timing.sfxt_ms = h->run_sfxt(false);
+    return 0;
+  });
+}
+
+int cpg_get_sfxt(cpg_handle* h, int* dist_out, int* succ_out, int* level_out) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (!h->loaded) throw std::runtime_error("no graph loaded");
+    if (!h->sfxt_valid) h->run_sfxt(false);
+    const size_t n = static_cast<size_t>(h->g.n);
+    if (dist_out) d2h(dist_out, h->dist.p, n, h->stream);
+    if (succ_out) d2h(succ_out, h->succ.p, n, h->stream);
+    if (level_out) d2h(level_out, h->level.p, n, h->stream);
+    CPG_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+  });
+}
+
+int cpg_cache_enable(cpg_handle* h, int enabled) {
+  CPG_CHECK_HANDLE(h, -1)
+  h->cache_enabled = enabled != 0;
+  return 0;
+}
+int cpg_cache_clear(cpg_handle* h) {
+  CPG_CHECK_HANDLE(h, -1)
+  h->invalidate_cache_payload();
+  h->cache_hits = h->cache_misses = 0;
+  return 0;
+}
+int cpg_cache_hits(cpg_handle* h) { return h ? h->cache_hits : -1; }
+int cpg_cache_misses(cpg_handle* h) { return h ? h->cache_misses : -1; }
+
+int cpg_gen_preset(int which, int index, int unit_wgts, cpg_edge_list* out) {
+  if (!out) { g_last_error = "null output"; return -1; }
+  return guarded<int>(-1, [&]() {
+    EdgeList el;
+    generate(preset(which, index, unit_wgts), el);
+    const size_t m = el.u.size();
+    out->n = el.n;
+    out->m = static_cast<int64_t>(m);
+    out->u = static_cast<int*>(std::malloc(sizeof(int) * (m ? m : 1)));
+    out->v = static_cast<int*>(std::malloc(sizeof(int) * (m ? m : 1)));
+    out->w = static_cast<float*>(std::malloc(sizeof(float) * (m ? m : 1)));
+    if (!out->u || !out->v || !out->w) throw std::runtime_error("out of memory");
+    std::memcpy(out->u, el.u.data(), sizeof(int) * m);
+    std::memcpy(out->v, el.v.data(), sizeof(int) * m);
+    std::memcpy(out->w, el.w.data(), sizeof(float) * m);
+    return 0;
+  });
+}
+
+void cpg_edge_list_free(cpg_edge_list* el) {
+  if (!el) return;
+  std::free(el->u);
+  std::free(el->v);
+  std::free(el->w);
+  el->u = el->v = nullptr;
+  el->w = nullptr;
+  el->n = 0;
+  el->m = 0;
+}
+
+int64_t cpg_preset_k(int which, int index) {
+  return guarded<int64_t>(-1, [&]() { return preset_k(which, index); });
+}
+
+int cpg_write_text(const cpg_edge_list* el, const char* path) {
+  if (!el || !path) { g_last_error = "null argument"; return -1; }
+  return guarded<int>(-1, [&]() {
+    EdgeList e;
+    e.n = el->n;
+    e.u.assign(el->u, el->u + el->m);
+    e.v.assign(el->v, el->v + el->m);
+    e.w.assign(el->w, el->w + el->m);
+    write_text(e, path);
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// Device-side argument blocks and host launch wrappers of the CPG kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cpg {
+
+// ---------------- SFXT ----------------
+struct SfxtCtl {
+  unsigned qtail;               // level-order queue tail
+  unsigned bar_count;           // grid barrier arrivals
+  unsigned bar_gen;             // grid barrier generation
+  int err;                      // barrier spin limit exceeded
+  int num_levels;               // graph diameter (levels peeled)
+  unsigned levelized;           // vertices peeled
+  unsigned max_abs_distf_bits;  // max |distf| as float bits
+  unsigned pad;
+};
+
+struct SfxtArgs {
+  int n;
+  const int* oadjp;
+  const int* oadj;
+  const float* owgt;
+  const int* iadjp;
+  const int* iadj;
+  int* deps;
+  int* queue;
+  int* lvlp;
+  int* level;
+  int* dist;
+  int* succ;
+  float* distf;
+  int4* vrec;      // {min delta bits, nsv, succ, distf bits}
+  int2* devinfo;   // {first deviation slot, deviation count}
+  int4* devs;      // {w, w_e bits, distf[w] bits, delta bits}, m slots
+  SfxtCtl* ctl;
+};
+
+void launch_sfxt(const SfxtArgs& args, hipStream_t stream, int* grid_out);
+
+// ---------------- PFXT ----------------
+struct PfxtCtl {
+  unsigned long long live_n;       // entries in the current live buffer
+  unsigned long long live_next_n;  // entries written by compaction
+  unsigned long long batch_n;      // parents selected this round
+  unsigned long long node_n;       // prefix-tree nodes stored
+  unsigned long long child_n;      // count pass: children <= bound
+  unsigned long long expanded;     // parents expanded, all rounds
+  unsigned long long hops;         // chain vertices visited (stats)
+  unsigned long long devvis;       // deviation records examined (stats)
+  unsigned long long fill_node_base;  // node_n before the fill pass
+  unsigned long long fill_live_base;  // live_n before the fill pass
+  unsigned long long node_cap;     // node store capacity
+  unsigned ukey;                   // bound: K-th smallest nonzero cost key (kKeyInf = none)
+  unsigned thkey;                  // batch threshold key
+  unsigned klo;                    // histogram window base key
+  unsigned shift;                  // histogram window shift
+  int done;                        // no parent left to expand
+  int overflow;                    // node capacity exceeded in a fill pass
+  int rounds;
+  int swap_pending;                // compaction output becomes the live buffer
+};
+
+struct NodeStore {
+  float* cost;
+  int* to;
+  int* from;
+  int* parent;
+  int* level;
+  uint64_t cap;
+};
+
+struct WalkArgs {
+  const int4* vrec;
+  const int2* devinfo;
+  const int4* devs;
+  const uint32_t* batch;
+  NodeStore nodes;
+  uint32_t* live_key;     // live buffer receiving appended children
+  uint32_t* live_idx;
+  unsigned* hist;         // count pass histogram (kHistBins)
+  PfxtCtl* ctl;
+  float max_abs_distf;
+  float max_abs_wgt;
+  int collect_stats;
+};
+
+struct LiveArgs {
+  const uint32_t* key;
+  uint32_t* idx;          // expanded flag is set in place
+  uint32_t* out_key;      // compaction target
+  uint32_t* out_idx;
+  uint32_t* batch;
+  unsigned* hist_all;
+  unsigned* hist_pend;
+  PfxtCtl* ctl;
+};
+
+void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dist,
+                  const float* distf, NodeStore nodes, uint32_t* live_key, uint32_t* live_idx,
+                  PfxtCtl* ctl, hipStream_t s);
+void launch_round_begin(PfxtCtl* ctl, hipStream_t s);
+void launch_hist_live(const LiveArgs& a, int grid, hipStream_t s);
+void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long batch_target,
+                         hipStream_t s);
+void launch_make_batch(const LiveArgs& a, int grid, hipStream_t s);
+void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s);
+void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
+                         hipStream_t s);
+void launch_compact_live(const LiveArgs& a, int grid, hipStream_t s);
+
+// ---------------- final selection ----------------
+// Stable LSD radix sort of (key, value) pairs over the given key bytes.
+struct SortBufs {
+  uint64_t* key[2];
+  uint32_t* val[2];
+  unsigned* ghist;        // 256 * max tiles
+  uint64_t cap;
+};
+int radix_sort_pairs(SortBufs& b, uint64_t n, const int* bytes, int nbytes, hipStream_t s);
+uint64_t sort_tiles(uint64_t n);
+
+void launch_build_sort_keys(const uint32_t* live_key, const uint32_t* live_idx, uint64_t n,
+                            const int* node_level, uint64_t* key, uint32_t* val,
+                            unsigned long long* zero_counts, hipStream_t s);
+void launch_emit_report(const uint64_t* key, const uint32_t* val, uint64_t nout, uint64_t nneg,
+                        uint64_t nzero, NodeStore nodes, float* ocost, int* olevel, int* ofrom,
+                        int* oto, int* opidx, int* rank_of, hipStream_t s);
+void launch_link_parents(const int* opidx, const int* rank_of, uint64_t nout, int* oparent,
+                         hipStream_t s);
+void launch_fill_int(int* p, int v, uint64_t n, hipStream_t s);
+
+}  // namespace cpg

@@ -1,0 +1,387 @@
+// PFXT: exact top-K prefix-tree expansion in bounded best-first rounds.
+//
+// A path is a root (a source followed by its suffix chain,
+// gpucpg/gpucpg.cu:18502-18513) or a parent plus one deviation edge u->w
+// taken at a vertex u of the parent's suffix chain, with cost
+//   ((parent + distf[w]) + w_e) - distf[u]
+// evaluated left to right in binary32 exactly as
+// compute_short_long_path_counts / expand_short_pile do
+// (gpucpg/gpucpg.cu:2670-2676, :3271-3276).
+//
+// Instead of the reference's short/long piles and split schedule
+// (gpucpg/gpucpg.cu:19134-21228), each round
+//   1. picks the cheapest unexpanded live paths (histogram select),
+//   2. walks their suffix chains once to histogram the children's costs
+//      (count pass),
+//   3. tightens the bound U to the K-th smallest nonzero cost among the
+//      live paths and those children,
+//   4. walks again and stores only children with cost <= U (fill pass),
+//   5. drops live paths above U.
+// U is always the K-th smallest cost of K distinct generated paths, so no
+// path of the final top K is ever dropped while child cost >= parent cost.
+// The walk skips chain segments via the deviation index built by the SFXT
+// kernel: a vertex whose smallest deviation delta exceeds
+// X = (U - parent) + slack is skipped through its next-smaller pointer and
+// its sorted deviation list is cut at the first delta > X.  slack bounds
+// the rounding gap between delta and the binary32 cost recurrence, so the
+// skip never hides a child whose computed cost is <= U; the exact
+// comparison cost <= U decides emission.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace cpg {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSelBlock = 1024;
+
+__device__ inline unsigned long long ld_u64(const unsigned long long* p) { return *p; }
+
+// Block-wide search for the first bin at which the running sum of h
+// reaches target (1-based count).  Returns kHistBins if the total is
+// smaller; *total receives the sum.  Must be called by all threads of a
+// kSelBlock block.
+__device__ uint32_t find_crossing(const unsigned* h, unsigned long long target,
+                                  unsigned long long* total) {
+  constexpr int kPer = kHistBins / kSelBlock;
+  __shared__ unsigned long long s_part[kSelBlock];
+  __shared__ unsigned s_bin;
+  const int t = threadIdx.x;
+  unsigned long long mine = 0;
+  for (int i = 0; i < kPer; ++i) mine += h[t * kPer + i];
+  s_part[t] = mine;
+  if (t == 0) s_bin = kHistBins;
+  __syncthreads();
+  // inclusive scan (Hillis-Steele) over kSelBlock partial sums
+  for (int off = 1; off < kSelBlock; off <<= 1) {
+    const unsigned long long v = t >= off ? s_part[t - off] : 0ull;
+    __syncthreads();
+    s_part[t] += v;
+    __syncthreads();
+  }
+  const unsigned long long incl = s_part[t];
+  const unsigned long long excl = incl - mine;
+  *total = s_part[kSelBlock - 1];
+  if (excl < target && incl >= target) {
+    unsigned long long run = excl;
+    for (int i = 0; i < kPer; ++i) {
+      run += h[t * kPer + i];
+      if (run >= target) { s_bin = static_cast<uint32_t>(t * kPer + i); break; }
+    }
+  }
+  __syncthreads();
+  const uint32_t b = s_bin;
+  __syncthreads();
+  return b;
+}
+
+__device__ inline void clear_hist(unsigned* h) {
+  for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) roots_kernel(const int* srcs, const uint8_t* keep, int nsrc,
+                                                       const int* dist, const float* distf,
+                                                       NodeStore nodes, uint32_t* live_key,
+                                                       uint32_t* live_idx, PfxtCtl* ctl) {
+  for (int base = blockIdx.x * blockDim.x; base < nsrc; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    bool take = false;
+    int s = 0;
+    if (i < nsrc) {
+      s = srcs[i];
+      take = keep[i] != 0 && dist[s] != kIntMax;
+    }
+    const uint64_t slot = wave_reserve(&ctl->node_n, take);
+    const uint64_t lslot = wave_reserve(&ctl->live_n, take);
+    if (take) {
+      const float c = distf[s];
+      nodes.cost[slot] = c;
+      nodes.to[slot] = s;
+      nodes.from[slot] = -1;
+      nodes.parent[slot] = -1;
+      nodes.level[slot] = 0;
+      live_key[lslot] = float_key(c);
+      live_idx[lslot] = static_cast<uint32_t>(slot);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) hist_live_kernel(LiveArgs a) {
+  PfxtCtl* ctl = a.ctl;
+  const uint64_t n = ctl->live_n;
+  const uint32_t klo = ctl->klo, shift = ctl->shift;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t k = a.key[i];
+    const uint32_t b = key_bin(k, klo, shift);
+    if (!key_is_zero(k)) atomicAdd(&a.hist_all[b], 1u);
+    if (!(a.idx[i] & kExpandedBit)) atomicAdd(&a.hist_pend[b], 1u);
+  }
+}
+
+__global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, unsigned* hist_pend,
+                                                                 unsigned long long target) {
+  if (ctl->done) return;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ctl->batch_n = 0;
+    ctl->child_n = 0;
+  }
+  unsigned long long total = 0;
+  const uint32_t b = find_crossing(hist_pend, target, &total);
+  if (threadIdx.x == 0) {
+    if (total == 0) {
+      ctl->done = 1;
+    } else if (b >= static_cast<uint32_t>(kHistBins)) {
+      ctl->thkey = ctl->ukey;
+    } else {
+      const uint32_t up = bin_upper_key(b, ctl->klo, ctl->shift);
+      ctl->thkey = up < ctl->ukey ? up : ctl->ukey;
+    }
+  }
+  __syncthreads();
+  clear_hist(hist_pend);
+}
+
+__global__ void __launch_bounds__(kBlock) make_batch_kernel(LiveArgs a) {
+  PfxtCtl* ctl = a.ctl;
+  if (ctl->done) return;
+  const uint64_t n = ctl->live_n;
+  const uint32_t th = ctl->thkey;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    bool take = false;
+    uint32_t id = 0;
+    if (i < n) {
+      id = a.idx[i];
+      take = !(id & kExpandedBit) && a.key[i] <= th;
+    }
+    const uint64_t slot = wave_reserve(&ctl->batch_n, take);
+    if (take) {
+      a.idx[i] = id | kExpandedBit;
+      a.batch[slot] = id;
+    }
+  }
+}
+
+template <bool kFill>
+__global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
+  PfxtCtl* ctl = a.ctl;
+  if (ctl->done) return;
+  const uint64_t nb = ctl->batch_n;
+  const uint32_t ukey = ctl->ukey;
+  const bool bounded = ukey != kKeyInf;
+  const float U = key_float(ukey);
+  const uint32_t klo = ctl->klo, shift = ctl->shift;
+  const uint64_t cap = a.nodes.cap;
+  const uint64_t node_base = kFill ? ctl->fill_node_base : 0;
+  const uint64_t live_base = kFill ? ctl->fill_live_base : 0;
+  const float mag = 2.0f * a.max_abs_distf + a.max_abs_wgt + 1.0f;
+  unsigned long long hops = 0, devv = 0, counted = 0;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < nb;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    int mode = 2;  // 0: at a chain vertex, 1: scanning its deviations, 2: done
+    int p = 0, v = -1, lvl = 0;
+    float c = 0.0f, X = __builtin_huge_valf();
+    if (i < nb) {
+      p = static_cast<int>(a.batch[i]);
+      c = a.nodes.cost[p];
+      v = a.nodes.to[p];
+      lvl = a.nodes.level[p];
+      if (bounded) X = (U - c) + ldexpf(fabsf(U) + fabsf(c) + mag, -17);
+      mode = v != -1 ? 0 : 2;
+    }
+    int j = 0, je = 0, sv = -1;
+    float dv = 0.0f;
+    while (__ballot(mode != 2)) {
+      bool emit = false;
+      float cc = 0.0f;
+      int cw = 0;
+      if (mode == 0) {
+        const int4 r = a.vrec[v];
+        ++hops;
+        if (__int_as_float(r.x) > X) {
+          v = r.y;
+          if (v == -1) mode = 2;
+        } else {
+          const int2 di = a.devinfo[v];
+          j = di.x;
+          je = di.x + di.y;
+          sv = r.z;
+          dv = __int_as_float(r.w);
+          mode = 1;
+        }
+      } else if (mode == 1) {
+        bool leave = j >= je;
+        if (!leave) {
+          const int4 d = a.devs[j];
+          ++devv;
+          if (__int_as_float(d.w) > X) {
+            leave = true;
+          } else {
+            cc = ((c + __int_as_float(d.z)) + __int_as_float(d.y)) - dv;
+            ++j;
+            if (cc <= U) {
+              emit = true;
+              cw = d.x;
+            }
+          }
+        }
+        if (leave) {
+          v = sv;
+          mode = v == -1 ? 2 : 0;
+        }
+      }
+      if (kFill) {
+        const uint64_t slot = wave_reserve(&ctl->node_n, emit);
+        if (emit) {
+          if (slot < cap) {
+            a.nodes.cost[slot] = cc;
+            a.nodes.to[slot] = cw;
+            a.nodes.from[slot] = v;
+            a.nodes.parent[slot] = p;
+            a.nodes.level[slot] = lvl + 1;
+            const uint64_t ls = live_base + (slot - node_base);
+            a.live_key[ls] = float_key(cc);
+            a.live_idx[ls] = static_cast<uint32_t>(slot);
+          } else {
+            atomicExch(&ctl->overflow, 1);
+          }
+        }
+      } else if (emit) {
+        ++counted;
+        const uint32_t k = float_key(cc);
+        if (!key_is_zero(k)) atomicAdd(&a.hist[key_bin(k, klo, shift)], 1u);
+      }
+    }
+  }
+  if (!kFill) {
+    for (int o = 32; o > 0; o >>= 1) counted += __shfl_xor(counted, o);
+    if (__lane_id() == 0 && counted) atomicAdd(&ctl->child_n, counted);
+  }
+  if (a.collect_stats) {
+    for (int o = 32; o > 0; o >>= 1) {
+      hops += __shfl_xor(hops, o);
+      devv += __shfl_xor(devv, o);
+    }
+    if (__lane_id() == 0) {
+      atomicAdd(&ctl->hops, hops);
+      atomicAdd(&ctl->devvis, devv);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kSelBlock) select_bound_kernel(PfxtCtl* ctl, unsigned* hist_all,
+                                                                 unsigned long long k) {
+  if (ctl->done) return;
+  unsigned long long total = 0;
+  const uint32_t b = find_crossing(hist_all, k, &total);
+  if (threadIdx.x == 0) {
+    uint32_t uk = ctl->ukey;
+    if (b < static_cast<uint32_t>(kHistBins)) {
+      const uint32_t up = bin_upper_key(b, ctl->klo, ctl->shift);
+      if (up < uk) uk = up;
+    }
+    ctl->ukey = uk;
+    ctl->shift = window_shift(ctl->klo, uk);
+    ctl->fill_node_base = ctl->node_n;
+    ctl->fill_live_base = ctl->live_n;
+    ctl->swap_pending = 1;
+    ctl->expanded += ctl->batch_n;
+    ctl->rounds += 1;
+  }
+  __syncthreads();
+  clear_hist(hist_all);
+}
+
+__global__ void __launch_bounds__(kBlock) compact_live_kernel(LiveArgs a) {
+  PfxtCtl* ctl = a.ctl;
+  if (ctl->done) return;
+  const uint64_t cap_nodes = ctl->node_cap;
+  const uint64_t nn = ctl->node_n < cap_nodes ? ctl->node_n : cap_nodes;
+  const uint64_t n = ctl->fill_live_base + (nn - ctl->fill_node_base);
+  const uint32_t uk = ctl->ukey;
+  const uint32_t klo = ctl->klo, shift = ctl->shift;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    bool keep = false;
+    uint32_t k = 0, id = 0;
+    if (i < n) {
+      k = a.key[i];
+      id = a.idx[i];
+      keep = k <= uk;
+    }
+    const uint64_t slot = wave_reserve(&ctl->live_next_n, keep);
+    if (keep) {
+      a.out_key[slot] = k;
+      a.out_idx[slot] = id;
+      const uint32_t b = key_bin(k, klo, shift);
+      if (!key_is_zero(k)) atomicAdd(&a.hist_all[b], 1u);
+      if (!(id & kExpandedBit)) atomicAdd(&a.hist_pend[b], 1u);
+    }
+  }
+}
+
+__global__ void round_begin_kernel(PfxtCtl* ctl) {
+  if (ctl->done) return;
+  if (ctl->swap_pending) {
+    ctl->live_n = ctl->live_next_n;
+    ctl->live_next_n = 0;
+    ctl->swap_pending = 0;
+  }
+}
+
+}  // namespace
+
+void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dist,
+                  const float* distf, NodeStore nodes, uint32_t* live_key, uint32_t* live_idx,
+                  PfxtCtl* ctl, hipStream_t s) {
+  if (nsrc <= 0) return;
+  const int grid = (nsrc + kBlock - 1) / kBlock;
+  roots_kernel<<<grid, kBlock, 0, s>>>(srcs, keep, nsrc, dist, distf, nodes, live_key, live_idx,
+                                       ctl);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_round_begin(PfxtCtl* ctl, hipStream_t s) {
+  round_begin_kernel<<<1, 1, 0, s>>>(ctl);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_hist_live(const LiveArgs& a, int grid, hipStream_t s) {
+  hist_live_kernel<<<grid, kBlock, 0, s>>>(a);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long target,
+                         hipStream_t s) {
+  select_batch_kernel<<<1, kSelBlock, 0, s>>>(ctl, hist_pend, target);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_make_batch(const LiveArgs& a, int grid, hipStream_t s) {
+  make_batch_kernel<<<grid, kBlock, 0, s>>>(a);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s) {
+  if (fill) walk_kernel<true><<<grid, kBlock, 0, s>>>(a);
+  else walk_kernel<false><<<grid, kBlock, 0, s>>>(a);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s) {
+  select_bound_kernel<<<1, kSelBlock, 0, s>>>(ctl, hist_all, k);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_compact_live(const LiveArgs& a, int grid, hipStream_t s) {
+  compact_live_kernel<<<grid, kBlock, 0, s>>>(a);
+  CPG_HIP(hipGetLastError());
+}
+
+}  // namespace cpg

@@ -1,0 +1,144 @@
+/*
+ * cpg.h -- C ABI of the MI355X critical-path-generation (CPG) engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (0 = ok, negative = error, message in cpg_last_error()).  No exception
+ * crosses this boundary.  One handle per thread; calls on one handle are
+ * serialised by the caller (the reference CpGen is not thread-safe either).
+ *
+ * Each entry point names the gpucpg::CpGen member it replaces
+ * (gpucpg/gpucpg.cuh, gpucpg/gpucpg.cu, paths relative to the reference
+ * repository root).  The C++ facade include/gpucpg_amd.hpp wraps this ABI
+ * in a class with the reference's method names.
+ */
+#ifndef CPG_AMD_H
+#define CPG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cpg_handle cpg_handle;
+
+/* PfxtNode, gpucpg/gpucpg.cuh:74-107 (24 bytes, same field order).
+ * parent is the 0-based rank of the parent path in the same report (the
+ * reference's parent index goes stale after its final sort,
+ * gpucpg/gpucpg.cu:3285 vs :21326); -1 for a root, or when the parent is
+ * not among the reported paths.  num_children is 0 as in the GPG mode's
+ * expand kernels (gpucpg/gpucpg.cu:3284-3288). */
+typedef struct {
+  int level;          /* number of deviation edges from the root path */
+  int from;           /* deviation vertex u (-1 for a root)           */
+  int to;             /* head of the path's suffix chain (source for roots) */
+  int parent;
+  int num_children;
+  float cost;         /* the reference's "slack"                      */
+} cpg_node;
+
+/* Query options.  Defaults (cpg_query_opts_default) are the GPG golden
+ * configuration of examples/tc-pfxt-inprocess-common.cuh:112-143. */
+typedef struct {
+  int struct_size;          /* sizeof(cpg_query_opts)                         */
+  float init_split_perc;    /* 0.005f: initial split percentile (gpucpg.cu:19140) */
+  int root_tie_quirk;       /* 1: drop roots tied at the initial split beyond
+                               (int)(p*n+1) as gpucpg.cu:19141-19180 does     */
+  int64_t batch;            /* parents expanded per round (0 = automatic)     */
+  int64_t max_nodes;        /* prefix-tree node capacity (0 = automatic)      */
+  int64_t mem_budget_bytes; /* device bytes a query may hold (0 = no limit)   */
+  const uint8_t* endpoint_mask; /* optional n-byte sink subset (NULL = all sinks) */
+  int collect_stats;        /* 1: count hops / deviations / emitted children  */
+} cpg_query_opts;
+
+/* Timings of the last query, milliseconds (gpucpg/gpucpg.cuh:327-356). */
+typedef struct {
+  double sfxt_ms;     /* levelize + relax + successor + deviation index (0 when cached) */
+  double pfxt_ms;     /* prefix-tree expansion rounds (reference-comparable pfxt_ms)     */
+  double select_ms;   /* final top-K sort and report assembly                            */
+  double query_ms;    /* whole device query, results resident in HBM                     */
+  int64_t rounds;     /* expansion rounds (the reference's "steps")                      */
+  int64_t expanded;   /* prefix-tree nodes expanded                                      */
+  int64_t generated;  /* prefix-tree nodes emitted (roots included)                      */
+  int64_t hops;       /* suffix-chain vertices visited (collect_stats)                   */
+  int64_t dev_visits; /* deviation records examined (collect_stats)                      */
+  int64_t roots;      /* roots seeded                                                    */
+  int64_t roots_dropped; /* roots lost to the initial-split tie truncation               */
+  int64_t retries;    /* rounds split after a node-capacity overflow                     */
+} cpg_timing;
+
+/* ---- lifetime ---- */
+cpg_handle* cpg_create(void);                      /* CpGen::CpGen   */
+void cpg_destroy(cpg_handle* h);                   /* CpGen::~CpGen  */
+const char* cpg_last_error(void);                  /* thread-local message */
+
+/* ---- graph input ---- */
+/* CpGen::read_input, gpucpg/gpucpg.cu:453-548. */
+int cpg_load_text(cpg_handle* h, const char* path, int ignore_wgts);
+/* Edge list in input order: same CSR semantics as the text loader. */
+int cpg_load_edges(cpg_handle* h, int n, int64_t m, const int* eu,
+                   const int* ev, const float* ew);
+/* Fanout CSR (adjp[n+1], adjncy[m], wgts[m]); the fanin CSR is derived in
+ * the same order as gpucpg/gpucpg.cu:531-546 would build it. */
+int cpg_load_csr(cpg_handle* h, int n, int64_t m, const int* adjp,
+                 const int* adjncy, const float* wgts);
+/* Copies of the host CSR the handle holds (NULL pointers are skipped). */
+int cpg_get_csr(cpg_handle* h, int* adjp, int* adjncy, float* wgts);
+
+int64_t cpg_num_verts(cpg_handle* h);              /* CpGen::num_verts, gpucpg.cuh:208 */
+int64_t cpg_num_edges(cpg_handle* h);              /* CpGen::num_edges, gpucpg.cuh:209 */
+int cpg_graph_diameter(cpg_handle* h);             /* CpGen::graph_diameter, gpucpg.cuh:348 */
+
+/* ---- query ---- */
+int cpg_reset(cpg_handle* h);                      /* CpGen::reset, gpucpg.cuh:211-250 */
+void cpg_query_opts_default(cpg_query_opts* o);
+/* CpGen::report_paths(k, 10, true, LEVELIZE_THEN_RELAX, SHORT_LONG, ...),
+ * gpucpg/gpucpg.cu:16910-21366: the k lowest-cost source->sink paths, exact,
+ * ascending cost, zero-cost paths excluded (gpucpg.cu:21338-21358). */
+int cpg_query(cpg_handle* h, int64_t k, const cpg_query_opts* o);
+int64_t cpg_num_paths(cpg_handle* h);              /* paths held by the last query */
+/* CpGen::get_slacks, gpucpg/gpucpg.cu:21409-21420: first k costs. */
+int64_t cpg_get_costs(cpg_handle* h, int64_t k, float* out);
+/* CpGen::get_pfxt_nodes, gpucpg/gpucpg.cu:21422-21440. */
+int64_t cpg_get_nodes(cpg_handle* h, int64_t k, cpg_node* out);
+/* Vertex sequence of the i-th reported path (source ... sink); returns the
+ * length, or a negative status (cap too small: -(length) - 1000).  No
+ * counterpart in the reference (its parents are stale after the sort). */
+int64_t cpg_get_path_vertices(cpg_handle* h, int64_t i, int* out, int64_t cap);
+int cpg_timings(cpg_handle* h, cpg_timing* out);
+
+/* ---- SFXT results (dist/succ/level of the last SFXT) ---- */
+/* dist: int-scaled suffix distance (relax_bu_step, gpucpg.cu:1167-1187),
+ * succ: update_successors_bu (gpucpg.cu:2606-2628), level: peel depth
+ * (bfs_td_step_privatized, gpucpg.cu:2215-2283).  Runs the SFXT if needed. */
+int cpg_sfxt(cpg_handle* h);
+int cpg_get_sfxt(cpg_handle* h, int* dist, int* succ, int* level);
+
+/* ---- static cache, gpucpg/gpucpg.cuh:186-189 ---- */
+int cpg_cache_enable(cpg_handle* h, int enabled);
+int cpg_cache_clear(cpg_handle* h);
+int cpg_cache_hits(cpg_handle* h);
+int cpg_cache_misses(cpg_handle* h);
+
+/* ---- synthetic workloads (host only; no device call) ---- */
+typedef struct {
+  int n;
+  int64_t m;
+  int* u;
+  int* v;
+  float* w;
+} cpg_edge_list;
+/* which: 1 = 50K-vertex/120-level DAG, 2 = 1.5M-vertex/300-level DAG,
+ * 3 = that DAG densified to mean out-degree 20, 4 = graph `index` of the
+ * 256-graph batch, 5 = graph `index` of the 8 served graphs. */
+int cpg_gen_preset(int which, int index, int unit_wgts, cpg_edge_list* out);
+void cpg_edge_list_free(cpg_edge_list* el);
+int64_t cpg_preset_k(int which, int index);
+/* Writes the benchmark text format read by cpg_load_text. */
+int cpg_write_text(const cpg_edge_list* el, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+"""GPU parity: the HIP engine against the oracle, through the C ABI.
+
+Bar: SFXT dist/succ/level equal element-for-element; path costs equal
+bit-for-bit at every prefix (sorted multiset for unit weights, where
+equal-cost ties make path identity ambiguous).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cpgamd
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+_CACHE = {}
+
+
+def preset_graph(which, unit=False):
+    key = (which, unit)
+    if key not in _CACHE:
+        n, u, v, w = cpgamd.gen_preset(which, 0, unit)
+        og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+        dist, succ, level, L = og.sfxt()
+        _CACHE[key] = (n, u, v, w, og, dist, succ, level, L)
+    return _CACHE[key]
+
+
+def engine_for(n, u, v, w):
+    g = cpgamd.CpGen()
+    g.load_edges(n, u, v, w)
+    return g
+
+
+def assert_bits_equal(want, got, what):
+    assert len(got) == len(want), f"{what}: {len(got)} paths, oracle {len(want)}"
+    mm = pyoracle.first_bit_mismatch(want, got)
+    assert mm == -1, f"{what}: first mismatch at rank {mm}: {got[mm]!r} vs {want[mm]!r}"
+
+
+@pytest.mark.parametrize("name", ["simple.txt", "dac21_paper_example.txt", "tv80_unit_wgts.txt"])
+def test_reference_benchmarks_sfxt_and_topk(golden_dir, name):
+    path = os.path.join(golden_dir, name)
+    say(f"case benchmark {name}")
+    og = pyoracle.OracleGraph.from_text(path)
+    dist, succ, level, L = og.sfxt()
+    g = cpgamd.CpGen()
+    g.read_input(path)
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    assert g.graph_diameter == L
+    for k in (1, 2, 5, 10, 100, 1000, 5000):
+        want, _ = og.topk(dist, succ, k)
+        g.reset()
+        g.report_paths(k)
+        got = g.get_slacks(k)
+        if name.startswith("tv80"):
+            np.testing.assert_array_equal(np.sort(got), np.sort(want))
+        else:
+            assert_bits_equal(want, got, f"{name} K={k}")
+
+
+def test_c1_sfxt_parity():
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1)
+    say("case C1 sfxt")
+    g = engine_for(n, u, v, w)
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    assert g.graph_diameter == L
+
+
+@pytest.mark.parametrize("k", [1, 100, 10000, 50000])
+def test_c1_topk_bit_exact(k):
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1)
+    say(f"case C1 topk K={k}")
+    want, st = og.topk(dist, succ, k)
+    assert st["nonmonotone"] == 0
+    g = engine_for(n, u, v, w)
+    g.report_paths(k)
+    got = g.get_slacks(k)
+    assert_bits_equal(want, got, f"C1 K={k}")
+
+
+def test_c2_sfxt_parity():
+    n, u, v, w, og, dist, succ, level, L = preset_graph(2)
+    say("case C2 sfxt")
+    g = engine_for(n, u, v, w)
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    assert g.graph_diameter == L
+
+
+def test_c2_topk_prefixes_bit_exact():
+    """K=1M on the 1.5M-vertex DAG; every prefix 1K/10K/50K/100K/1M."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(2)
+    say("case C2 oracle K=1M")
+    want, st = og.topk(dist, succ, 1_000_000)
+    assert st["nonmonotone"] == 0
+    say("case C2 GPU K=1M")
+    g = engine_for(n, u, v, w)
+    g.report_paths(1_000_000)
+    got = g.get_slacks(1_000_000)
+    say(f"C2 timings {g.timings()}")
+    for kp in (1000, 10_000, 50_000, 100_000, 1_000_000):
+        assert_bits_equal(want[:kp], got[:kp], f"C2 prefix {kp}")
