@@ -25,7 +25,8 @@ class QueryOpts(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_int), ("init_split_perc", ctypes.c_float),
                 ("root_tie_quirk", ctypes.c_int), ("batch", ctypes.c_int64),
                 ("max_nodes", ctypes.c_int64), ("mem_budget_bytes", ctypes.c_int64),
-                ("endpoint_mask", ctypes.c_void_p), ("collect_stats", ctypes.c_int)]
+                ("endpoint_mask", ctypes.c_void_p), ("collect_stats", ctypes.c_int),
+                ("kernel_timing", ctypes.c_int)]
 
 
 class Timing(ctypes.Structure):
@@ -34,7 +35,9 @@ class Timing(ctypes.Structure):
                 ("rounds", ctypes.c_int64), ("expanded", ctypes.c_int64),
                 ("generated", ctypes.c_int64), ("hops", ctypes.c_int64),
                 ("dev_visits", ctypes.c_int64), ("roots", ctypes.c_int64),
-                ("roots_dropped", ctypes.c_int64), ("retries", ctypes.c_int64)]
+                ("roots_dropped", ctypes.c_int64), ("retries", ctypes.c_int64),
+                ("sfxt_kernel_ms", ctypes.c_double), ("walk_count_ms", ctypes.c_double),
+                ("walk_fill_ms", ctypes.c_double), ("walk_launches", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -205,7 +208,8 @@ class CpGen:
         _check(lib().cpg_reset(self.h))
 
     def report_paths(self, k, *args, root_tie_quirk=True, batch=0, max_nodes=0,
-                     collect_stats=False, init_split_perc=0.005, **kwargs):
+                     collect_stats=False, init_split_perc=0.005, kernel_timing=False,
+                     **kwargs):
         """Runs the GPG query for the k lowest-cost paths.  Positional
         arguments after k (max_dev_lvls, enable_compress, methods, ...) are
         accepted for call compatibility with the reference; the engine always
@@ -217,6 +221,7 @@ class CpGen:
         o.max_nodes = int(max_nodes)
         o.collect_stats = 1 if collect_stats else 0
         o.init_split_perc = float(init_split_perc)
+        o.kernel_timing = 1 if kernel_timing else 0
         _check(lib().cpg_query(self.h, int(k), ctypes.byref(o)))
         self._k = int(k)
 

@@ -64,6 +64,24 @@ void d2h(T* h, const T* d, size_t n, hipStream_t s) {
   if (n) CPG_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
 }
 
+// A pair of HIP events bracketing work on one stream.
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  void create() {
+    if (!a) CPG_HIP(hipEventCreate(&a));
+    if (!b) CPG_HIP(hipEventCreate(&b));
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  float ms() const {
+    float t = 0.0f;
+    CPG_HIP(hipEventElapsedTime(&t, a, b));
+    return t;
+  }
+};
+
 }  // namespace
 
 struct cpg_handle {
@@ -113,6 +131,8 @@ struct cpg_handle {
   bool host_result = false;
 
   cpg_timing timing{};
+  bool kernel_timing = false;
+  EventPair ev_sfxt, ev_count, ev_fill;
 
   cpg_handle() {}
   ~cpg_handle() {
@@ -191,11 +211,14 @@ struct cpg_handle {
       a.deps = deps.p; a.queue = queue.p; a.lvlp = lvlp.p; a.level = level.p;
       a.dist = dist.p; a.succ = succ.p; a.distf = distf.p;
       a.vrec = vrec.p; a.devinfo = devinfo.p; a.devs = devs.p; a.ctl = sctl.p;
+      if (kernel_timing) { ev_sfxt.create(); CPG_HIP(hipEventRecord(ev_sfxt.a, stream)); }
       launch_sfxt(a, stream, nullptr);
+      if (kernel_timing) CPG_HIP(hipEventRecord(ev_sfxt.b, stream));
     }
     SfxtCtl c{};
     d2h(&c, sctl.p, 1, stream);
     CPG_HIP(hipStreamSynchronize(stream));
+    if (kernel_timing && n > 0) timing.sfxt_kernel_ms = ev_sfxt.ms();
     if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
     diameter = c.num_levels;
     uint32_t bits = c.max_abs_distf_bits;
@@ -223,6 +246,7 @@ struct cpg_handle {
     host_result = false;
     nres = 0;
     timing = cpg_timing{};
+    kernel_timing = o.kernel_timing != 0;
     const double tq0 = now_ms();
     timing.sfxt_ms = run_sfxt(true);
     if (k <= 0 || g.n == 0) {
@@ -313,12 +337,25 @@ struct cpg_handle {
         launch_make_batch(l, grid, stream);
         wa.live_key = live_key[cur].p;
         wa.live_idx = live_idx[cur].p;
+        if (kernel_timing) {
+          ev_count.create();
+          ev_fill.create();
+          CPG_HIP(hipEventRecord(ev_count.a, stream));
+        }
         launch_walk(wa, false, grid, stream);
+        if (kernel_timing) CPG_HIP(hipEventRecord(ev_count.b, stream));
         launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+        if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.a, stream));
         launch_walk(wa, true, grid, stream);
+        if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
         launch_compact_live(l, grid, stream);
         d2h(&c, pctl.p, 1, stream);
         CPG_HIP(hipStreamSynchronize(stream));
+        if (kernel_timing) {
+          timing.walk_count_ms += ev_count.ms();
+          timing.walk_fill_ms += ev_fill.ms();
+          timing.walk_launches += 1;
+        }
         if (c.overflow) { overflow = true; break; }
         if (c.done) break;  // the round did nothing: live stays in `cur`
         cur ^= 1;

@@ -50,6 +50,8 @@ typedef struct {
   int64_t mem_budget_bytes; /* device bytes a query may hold (0 = no limit)   */
   const uint8_t* endpoint_mask; /* optional n-byte sink subset (NULL = all sinks) */
   int collect_stats;        /* 1: count hops / deviations / emitted children  */
+  int kernel_timing;        /* 1: time the SFXT and walk kernels with HIP events
+                               on the engine's stream (cpg_timing.*_kernel_ms) */
 } cpg_query_opts;
 
 /* Timings of the last query, milliseconds (gpucpg/gpucpg.cuh:327-356). */
@@ -65,7 +67,11 @@ typedef struct {
   int64_t dev_visits; /* deviation records examined (collect_stats)                      */
   int64_t roots;      /* roots seeded                                                    */
   int64_t roots_dropped; /* roots lost to the initial-split tie truncation               */
-  int64_t retries;    /* rounds split after a node-capacity overflow                     */
+  int64_t retries;    /* query restarts after a node-capacity overflow                   */
+  double sfxt_kernel_ms;  /* kernel_timing: SFXT kernel, HIP events                      */
+  double walk_count_ms;   /* kernel_timing: count-pass walk kernels, summed              */
+  double walk_fill_ms;    /* kernel_timing: fill-pass walk kernels, summed               */
+  int64_t walk_launches;  /* kernel_timing: walk kernel launches of each kind            */
 } cpg_timing;
 
 /* ---- lifetime ---- */

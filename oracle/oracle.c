@@ -274,6 +274,31 @@ float orc_split_inc_amount(float avg_deg) {
   return mn + (mx - mn) / (1 + expf(k * (avg_deg - d0)));
 }
 
+int orc_classify(float slack, float split, float final_split, int use_final_split,
+                 int skip_long_paths) {
+  if (slack <= split) return 0;
+  if (!skip_long_paths && (!use_final_split || slack <= final_split)) return 1;
+  return 2;
+}
+
+long long orc_deviations(int n, const int* adjp, const int* adj, const float* wgt,
+                         const int* succ, const int* dist, int* offsets, int* dsts,
+                         float* deltas) {
+  long long c = 0;
+  offsets[0] = 0;
+  for (int u = 0; u < n; ++u) {
+    for (int e = adjp[u]; e < adjp[u + 1]; ++e) {
+      const int w = adj[e];
+      if (w == succ[u] || dist[w] == ORC_INT_MAX) continue;
+      dsts[c] = w;
+      deltas[c] = (float)dist[w] / (float)ORC_SCALE_UP + wgt[e] - (float)dist[u] / (float)ORC_SCALE_UP;
+      ++c;
+    }
+    offsets[u + 1] = (int)c;
+  }
+  return c;
+}
+
 /* ------------------------------------------------------------------ */
 /* heaps                                                              */
 /* ------------------------------------------------------------------ */
@@ -620,14 +645,14 @@ long long orc_short_long(const orc_graph* g, const int* dist,
     long long wl = 0, ws = 0, n_le = 0;
     for (long long i = 0; i < short_size; ++i) {
       if (sp.a[i].slack > split0) {
-        if (wl >= num_long) { st->aborted = 1; goto out; }  /* reference UB */
+        if (wl >= num_long) { st->aborted = 3; goto out; }  /* reference UB */
         lp.a[wl++] = sp.a[i];
       } else {
         sp.a[ws++] = sp.a[i];
         ++n_le;
       }
     }
-    if (n_le < num_short) { st->aborted = 1; goto out; }  /* reference UB */
+    if (n_le < num_short) { st->aborted = 3; goto out; }  /* reference UB */
     st->roots_dropped = n_le - num_short;
     st->long_dummies = num_long - wl;
     sp.n = num_short;
@@ -653,7 +678,7 @@ long long orc_short_long(const orc_graph* g, const int* dist,
             final_window_size == 0 && short_size + ns < k) {
           const long long needed = k - (short_size + ns);
           qsort(lp.a, (size_t)lp.n, sizeof(orc_node), cmp_node);
-          if (needed - 1 >= lp.n) { st->aborted = 1; goto out; }
+          if (needed - 1 >= lp.n) { st->aborted = 2; goto out; }
           final_split = lp.a[needed - 1].slack;
           long long w2 = 0;
           for (long long i = 0; i < lp.n; ++i)
@@ -661,6 +686,7 @@ long long orc_short_long(const orc_graph* g, const int* dist,
           lp.n = w2;
           final_window_size = w2;
           long_size = w2;
+          st->final_window = w2;
         }
         short_size += ns;
         if (short_size + long_size + nl > node_budget) { st->aborted = 1; goto out; }
@@ -701,7 +727,7 @@ long long orc_short_long(const orc_graph* g, const int* dist,
         ++steps;
         if (long_size == 0 || short_size >= k) break;
         while (ns == 0) {
-          if (++guard > 100000000) { st->aborted = 1; goto out; }
+          if (++guard > 100000000) { st->aborted = 4; goto out; }
           if (steps == 1) {
             float mn = FLT_MAX;
             for (long long i = 0; i < lp.n; ++i) if (lp.a[i].slack < mn) mn = lp.a[i].slack;

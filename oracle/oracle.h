@@ -18,6 +18,8 @@
  *   orc_successors      brute-force max-id arg-min per vertex.
  *   orc_distf           IEEE float division (numpy float32 divide).
  *   orc_candidate_cost  known answer -0.75 of unittests/tc_pfxt_candidates.cu:85-90.
+ *   orc_classify        known answers of unittests/tc_pfxt_candidates.cu:69-83.
+ *   orc_deviations      known answer of unittests/tc_pfxt_candidates.cu:386-408.
  *   orc_topk            brute-force enumeration of every source->sink path
  *                       (orc_all_paths) on tiny DAGs; the faithful SEQUENTIAL
  *                       and SHORT_LONG transliterations below on small DAGs.
@@ -73,8 +75,11 @@ typedef struct {
   long long long_dummies;    /* default slack-0 nodes left in the long pile */
   long long steps;           /* SHORT_LONG steps                          */
   long long short_pile;      /* SHORT_LONG final short pile size          */
+  long long final_window;    /* SHORT_LONG prefetched final window size (0 = unused) */
   int split_index_mismatch;  /* (int)(p*n+1) != (size_t)(p*n)+1           */
-  int aborted;               /* node budget exceeded                      */
+  int aborted;               /* 0 ok; 1 node budget; 2 final-window index past the
+                                long pile; 3 initial split out of range; 4 split
+                                update made no progress (reference UB or hang) */
 } orc_stats;
 
 /* ---- input (a1) ---- */
@@ -97,6 +102,18 @@ int orc_sfxt(const orc_graph* g, int* dist, int* succ, int* level,
 /* ---- candidate cost (a10/a17) ---- */
 float orc_candidate_cost(float parent, int src_dist, int dst_dist, float wgt);
 float orc_split_inc_amount(float avg_deg);
+/* classify_candidate, gpucpg/tc_pfxt_candidates.cuh:905-918:
+ * 0 = short (<= split), 1 = long, 2 = skip. */
+int orc_classify(float slack, float split, float final_split, int use_final_split,
+                 int skip_long_paths);
+/* Deviation edges of every vertex (u->w with w != succ[u] and dist[w]
+ * finite, gpucpg/gpucpg.cu:2662-2668, tc_pfxt_candidates.cuh:264-276) in
+ * fanout order, with delta = distf[w] + w_e - distf[u] evaluated in that
+ * order in binary32 (tc_pfxt_candidates.cuh:662-669).  offsets has n+1
+ * entries; returns the number of deviations written. */
+long long orc_deviations(int n, const int* adjp, const int* adj, const float* wgt,
+                         const int* succ, const int* dist, int* offsets, int* dsts,
+                         float* deltas);
 
 /* ---- PFXT (a7-a15) ---- */
 long long orc_topk(const orc_graph* g, const int* dist, const int* succ,

@@ -24,7 +24,7 @@ class Graph(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_longlong) for name in (
         "pops", "pushes", "hops", "edges", "children", "nonmonotone", "zero_cost", "roots",
-        "roots_dropped", "long_dummies", "steps", "short_pile")] + [
+        "roots_dropped", "long_dummies", "steps", "short_pile", "final_window")] + [
         ("split_index_mismatch", ctypes.c_int), ("aborted", ctypes.c_int)]
 
     def as_dict(self):
@@ -55,6 +55,9 @@ _SIGS = {
     "orc_sfxt": (ctypes.c_int, [_PG, _PI, _PI, _PI, _PI]),
     "orc_candidate_cost": (ctypes.c_float, [ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float]),
     "orc_split_inc_amount": (ctypes.c_float, [ctypes.c_float]),
+    "orc_classify": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                    ctypes.c_int]),
+    "orc_deviations": (_LL, [ctypes.c_int, _PI, _PI, _PF, _PI, _PI, _PI, _PI, _PF]),
     "orc_topk": (_LL, [_PG, _PI, _PI, _LL, ctypes.c_int, _PF, _PS]),
     "orc_sequential": (_LL, [_PG, _PI, _PI, _LL, _PF, _PS]),
     "orc_short_long": (_LL, [_PG, _PI, _PI, _LL, ctypes.c_float, ctypes.c_int, ctypes.c_float,
@@ -68,6 +71,15 @@ _SIGS = {
                                        ctypes.POINTER(ctypes.c_double)]),
 }
 _lib = None
+
+
+class ShortLongAbort(RuntimeError):
+    """The schedule reached a state where the reference's own code is
+    undefined (st["aborted"] says which) or the node budget ran out."""
+
+    def __init__(self, stats):
+        super().__init__("oracle short_long aborted: %r" % stats)
+        self.stats = stats
 
 
 def lib():
@@ -183,7 +195,7 @@ class OracleGraph:
                                    float(long_pile_limit_bytes), int(node_budget), _fp(out),
                                    ctypes.byref(st))
         if got < 0:
-            raise RuntimeError("oracle short_long aborted: %r" % st.as_dict())
+            raise ShortLongAbort(st.as_dict())
         return out[:got], st.as_dict()
 
     def all_paths(self, dist, succ, cap=1_000_000):
@@ -233,3 +245,22 @@ def first_bit_mismatch(a, b):
     b = np.ascontiguousarray(b, np.float32)
     n = min(len(a), len(b))
     return lib().orc_first_bit_mismatch(_fp(a), _fp(b), n)
+
+
+def classify(slack, split, final_split, use_final_split, skip_long_paths):
+    return lib().orc_classify(slack, split, final_split, int(use_final_split), int(skip_long_paths))
+
+
+def deviations(adjp, adj, wgt, succ, dist):
+    adjp = np.ascontiguousarray(adjp, np.int32)
+    adj = np.ascontiguousarray(adj, np.int32)
+    wgt = np.ascontiguousarray(wgt, np.float32)
+    succ = np.ascontiguousarray(succ, np.int32)
+    dist = np.ascontiguousarray(dist, np.int32)
+    n = len(adjp) - 1
+    offs = np.zeros(n + 1, np.int32)
+    dsts = np.zeros(max(len(adj), 1), np.int32)
+    dl = np.zeros(max(len(adj), 1), np.float32)
+    c = lib().orc_deviations(n, _ip(adjp), _ip(adj), _fp(wgt), _ip(succ), _ip(dist), _ip(offs),
+                             _ip(dsts), _fp(dl))
+    return offs, dsts[:c], dl[:c]
