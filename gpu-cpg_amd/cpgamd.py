@@ -89,18 +89,24 @@ ABI = {
 _lib = None
 
 
+def load_library(path):
+    """Loads a build of the C ABI from an explicit path and declares its
+    signatures.  The default (lib()) is the HIP build next to this file."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in ABI.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
 def lib():
-    """Loads libcpgamd.so once; raises if it is missing."""
+    """The HIP engine library; raises if it is missing."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
-        L = ctypes.CDLL(LIB_PATH)
-        for name, (res, args) in ABI.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
 
 
@@ -108,9 +114,9 @@ class CpgError(RuntimeError):
     pass
 
 
-def _check(rc):
+def _check(rc, L=None):
     if rc is None or (isinstance(rc, int) and rc < 0):
-        raise CpgError(lib().cpg_last_error().decode())
+        raise CpgError((L or lib()).cpg_last_error().decode())
     return rc
 
 
@@ -148,18 +154,23 @@ def write_text(n, u, v, w, path):
     _check(lib().cpg_write_text(ctypes.byref(el), path.encode()))
 
 
+def _c(g, rc):
+    return _check(rc, g.L)
+
+
 class CpGen:
     """gpucpg::CpGen-shaped handle over the C ABI."""
 
-    def __init__(self):
-        self.h = lib().cpg_create()
+    def __init__(self, library=None):
+        self.L = library or lib()
+        self.h = self.L.cpg_create()
         if not self.h:
-            raise CpgError(lib().cpg_last_error().decode())
+            raise CpgError(self.L.cpg_last_error().decode())
         self._k = 0
 
     def close(self):
         if self.h:
-            lib().cpg_destroy(self.h)
+            self.L.cpg_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -170,19 +181,19 @@ class CpGen:
 
     # -- input (gpucpg/gpucpg.cu:453-548)
     def read_input(self, filename, ignore_wgts=False):
-        _check(lib().cpg_load_text(self.h, filename.encode(), 1 if ignore_wgts else 0))
+        _c(self, self.L.cpg_load_text(self.h, filename.encode(), 1 if ignore_wgts else 0))
 
     def load_edges(self, n, u, v, w):
         u = np.ascontiguousarray(u, np.int32)
         v = np.ascontiguousarray(v, np.int32)
         w = np.ascontiguousarray(w, np.float32)
-        _check(lib().cpg_load_edges(self.h, int(n), len(u), _iptr(u), _iptr(v), _fptr(w)))
+        _c(self, self.L.cpg_load_edges(self.h, int(n), len(u), _iptr(u), _iptr(v), _fptr(w)))
 
     def load_csr(self, adjp, adjncy, wgts):
         adjp = np.ascontiguousarray(adjp, np.int32)
         adjncy = np.ascontiguousarray(adjncy, np.int32)
         wgts = np.ascontiguousarray(wgts, np.float32)
-        _check(lib().cpg_load_csr(self.h, len(adjp) - 1, len(adjncy), _iptr(adjp),
+        _c(self, self.L.cpg_load_csr(self.h, len(adjp) - 1, len(adjncy), _iptr(adjp),
                                   _iptr(adjncy), _fptr(wgts)))
 
     def csr(self):
@@ -190,22 +201,22 @@ class CpGen:
         adjp = np.zeros(n + 1, np.int32)
         adj = np.zeros(max(m, 1), np.int32)
         w = np.zeros(max(m, 1), np.float32)
-        _check(lib().cpg_get_csr(self.h, _iptr(adjp), _iptr(adj), _fptr(w)))
+        _c(self, self.L.cpg_get_csr(self.h, _iptr(adjp), _iptr(adj), _fptr(w)))
         return adjp, adj[:m], w[:m]
 
     def num_verts(self):
-        return lib().cpg_num_verts(self.h)
+        return self.L.cpg_num_verts(self.h)
 
     def num_edges(self):
-        return lib().cpg_num_edges(self.h)
+        return self.L.cpg_num_edges(self.h)
 
     @property
     def graph_diameter(self):
-        return lib().cpg_graph_diameter(self.h)
+        return self.L.cpg_graph_diameter(self.h)
 
     # -- query (gpucpg/gpucpg.cuh:211-250, gpucpg/gpucpg.cu:16910-21366)
     def reset(self):
-        _check(lib().cpg_reset(self.h))
+        _c(self, self.L.cpg_reset(self.h))
 
     def report_paths(self, k, *args, root_tie_quirk=True, batch=0, max_nodes=0,
                      collect_stats=False, init_split_perc=0.005, kernel_timing=False,
@@ -215,28 +226,28 @@ class CpGen:
         accepted for call compatibility with the reference; the engine always
         runs its exact GPU path."""
         o = QueryOpts()
-        lib().cpg_query_opts_default(ctypes.byref(o))
+        self.L.cpg_query_opts_default(ctypes.byref(o))
         o.root_tie_quirk = 1 if root_tie_quirk else 0
         o.batch = int(batch)
         o.max_nodes = int(max_nodes)
         o.collect_stats = 1 if collect_stats else 0
         o.init_split_perc = float(init_split_perc)
         o.kernel_timing = 1 if kernel_timing else 0
-        _check(lib().cpg_query(self.h, int(k), ctypes.byref(o)))
+        _c(self, self.L.cpg_query(self.h, int(k), ctypes.byref(o)))
         self._k = int(k)
 
     def num_paths(self):
-        return lib().cpg_num_paths(self.h)
+        return self.L.cpg_num_paths(self.h)
 
     def get_slacks(self, k):
         out = np.zeros(max(int(k), 1), np.float32)
-        n = _check(lib().cpg_get_costs(self.h, int(k), _fptr(out)))
+        n = _c(self, self.L.cpg_get_costs(self.h, int(k), _fptr(out)))
         return out[:n]
 
     def get_pfxt_nodes(self, k):
         k = max(int(k), 0)
         buf = (CpgNode * max(k, 1))()
-        n = _check(lib().cpg_get_nodes(self.h, k, buf))
+        n = _c(self, self.L.cpg_get_nodes(self.h, k, buf))
         arr = np.ctypeslib.as_array(buf)[:n]
         return np.array(arr, copy=True)
 
@@ -244,17 +255,17 @@ class CpGen:
         cap = 1 << 16
         while True:
             out = np.zeros(cap, np.int32)
-            n = lib().cpg_get_path_vertices(self.h, int(i), _iptr(out), cap)
+            n = self.L.cpg_get_path_vertices(self.h, int(i), _iptr(out), cap)
             if n >= 0:
                 return out[:n]
             if n <= -1000:
                 cap = -(n + 1000) + 1
                 continue
-            raise CpgError(lib().cpg_last_error().decode())
+            raise CpgError(self.L.cpg_last_error().decode())
 
     def timings(self):
         t = Timing()
-        _check(lib().cpg_timings(self.h, ctypes.byref(t)))
+        _c(self, self.L.cpg_timings(self.h, ctypes.byref(t)))
         return t.as_dict()
 
     def sfxt(self):
@@ -262,21 +273,21 @@ class CpGen:
         dist = np.zeros(max(n, 1), np.int32)
         succ = np.zeros(max(n, 1), np.int32)
         level = np.zeros(max(n, 1), np.int32)
-        _check(lib().cpg_get_sfxt(self.h, _iptr(dist), _iptr(succ), _iptr(level)))
+        _c(self, self.L.cpg_get_sfxt(self.h, _iptr(dist), _iptr(succ), _iptr(level)))
         return dist[:n], succ[:n], level[:n]
 
     def run_sfxt(self):
-        _check(lib().cpg_sfxt(self.h))
+        _c(self, self.L.cpg_sfxt(self.h))
 
     # -- static cache (gpucpg/gpucpg.cuh:186-189)
     def enable_tc_pfxt_static_cache(self, enabled):
-        _check(lib().cpg_cache_enable(self.h, 1 if enabled else 0))
+        _c(self, self.L.cpg_cache_enable(self.h, 1 if enabled else 0))
 
     def clear_tc_pfxt_static_cache(self):
-        _check(lib().cpg_cache_clear(self.h))
+        _c(self, self.L.cpg_cache_clear(self.h))
 
     def tc_pfxt_static_cache_hits(self):
-        return lib().cpg_cache_hits(self.h)
+        return self.L.cpg_cache_hits(self.h)
 
     def tc_pfxt_static_cache_misses(self):
-        return lib().cpg_cache_misses(self.h)
+        return self.L.cpg_cache_misses(self.h)

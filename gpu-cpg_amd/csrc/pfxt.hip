@@ -342,45 +342,45 @@ void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dis
                   PfxtCtl* ctl, hipStream_t s) {
   if (nsrc <= 0) return;
   const int grid = (nsrc + kBlock - 1) / kBlock;
-  roots_kernel<<<grid, kBlock, 0, s>>>(srcs, keep, nsrc, dist, distf, nodes, live_key, live_idx,
+  hipLaunchKernelGGL(roots_kernel, dim3(grid), dim3(kBlock), 0, s, srcs, keep, nsrc, dist, distf, nodes, live_key, live_idx,
                                        ctl);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_round_begin(PfxtCtl* ctl, hipStream_t s) {
-  round_begin_kernel<<<1, 1, 0, s>>>(ctl);
+  hipLaunchKernelGGL(round_begin_kernel, dim3(1), dim3(1), 0, s, ctl);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_hist_live(const LiveArgs& a, int grid, hipStream_t s) {
-  hist_live_kernel<<<grid, kBlock, 0, s>>>(a);
+  hipLaunchKernelGGL(hist_live_kernel, dim3(grid), dim3(kBlock), 0, s, a);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long target,
                          hipStream_t s) {
-  select_batch_kernel<<<1, kSelBlock, 0, s>>>(ctl, hist_pend, target);
+  hipLaunchKernelGGL(select_batch_kernel, dim3(1), dim3(kSelBlock), 0, s, ctl, hist_pend, target);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_make_batch(const LiveArgs& a, int grid, hipStream_t s) {
-  make_batch_kernel<<<grid, kBlock, 0, s>>>(a);
+  hipLaunchKernelGGL(make_batch_kernel, dim3(grid), dim3(kBlock), 0, s, a);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s) {
-  if (fill) walk_kernel<true><<<grid, kBlock, 0, s>>>(a);
-  else walk_kernel<false><<<grid, kBlock, 0, s>>>(a);
+  if (fill) hipLaunchKernelGGL((walk_kernel<true>), dim3(grid), dim3(kBlock), 0, s, a);
+  else hipLaunchKernelGGL((walk_kernel<false>), dim3(grid), dim3(kBlock), 0, s, a);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s) {
-  select_bound_kernel<<<1, kSelBlock, 0, s>>>(ctl, hist_all, k);
+  hipLaunchKernelGGL(select_bound_kernel, dim3(1), dim3(kSelBlock), 0, s, ctl, hist_all, k);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_compact_live(const LiveArgs& a, int grid, hipStream_t s) {
-  compact_live_kernel<<<grid, kBlock, 0, s>>>(a);
+  hipLaunchKernelGGL(compact_live_kernel, dim3(grid), dim3(kBlock), 0, s, a);
   CPG_HIP(hipGetLastError());
 }
 

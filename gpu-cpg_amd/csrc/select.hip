@@ -180,11 +180,11 @@ int radix_sort_pairs(SortBufs& b, uint64_t n, const int* bytes, int nbytes, hipS
   const uint64_t ntiles = sort_tiles(n);
   for (int p = 0; p < nbytes; ++p) {
     const int shift = 8 * bytes[p];
-    sort_hist_kernel<<<static_cast<unsigned>(ntiles), kSortBlock, 0, s>>>(b.key[cur], n, shift, b.ghist, ntiles);
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(static_cast<unsigned>(ntiles)), dim3(kSortBlock), 0, s, b.key[cur], n, shift, b.ghist, ntiles);
     CPG_HIP(hipGetLastError());
-    sort_scan_kernel<<<1, kScanBlock, 0, s>>>(b.ghist, 256 * ntiles);
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, b.ghist, 256 * ntiles);
     CPG_HIP(hipGetLastError());
-    sort_scatter_kernel<<<static_cast<unsigned>(ntiles), kSortBlock, 0, s>>>(
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3(static_cast<unsigned>(ntiles)), dim3(kSortBlock), 0, s, 
         b.key[cur], b.val[cur], b.key[cur ^ 1], b.val[cur ^ 1], n, shift, b.ghist, ntiles);
     CPG_HIP(hipGetLastError());
     cur ^= 1;
@@ -196,7 +196,7 @@ void launch_build_sort_keys(const uint32_t* live_key, const uint32_t* live_idx, 
                             const int* node_level, uint64_t* key, uint32_t* val,
                             unsigned long long* zero_counts, hipStream_t s) {
   if (n == 0) return;
-  build_keys_kernel<<<grid_for(n, 256), 256, 0, s>>>(live_key, live_idx, n, node_level, key, val,
+  hipLaunchKernelGGL(build_keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, live_key, live_idx, n, node_level, key, val,
                                                      zero_counts);
   CPG_HIP(hipGetLastError());
 }
@@ -205,7 +205,7 @@ void launch_emit_report(const uint64_t* /*key*/, const uint32_t* val, uint64_t n
                         uint64_t nzero, NodeStore nodes, float* ocost, int* olevel, int* ofrom,
                         int* oto, int* opidx, int* rank_of, hipStream_t s) {
   if (nout == 0) return;
-  emit_report_kernel<<<grid_for(nout, 256), 256, 0, s>>>(val, nout, nneg, nzero, nodes, ocost,
+  hipLaunchKernelGGL(emit_report_kernel, dim3(grid_for(nout, 256)), dim3(256), 0, s, val, nout, nneg, nzero, nodes, ocost,
                                                          olevel, ofrom, oto, opidx, rank_of);
   CPG_HIP(hipGetLastError());
 }
@@ -213,13 +213,13 @@ void launch_emit_report(const uint64_t* /*key*/, const uint32_t* val, uint64_t n
 void launch_link_parents(const int* opidx, const int* rank_of, uint64_t nout, int* oparent,
                          hipStream_t s) {
   if (nout == 0) return;
-  link_parents_kernel<<<grid_for(nout, 256), 256, 0, s>>>(opidx, rank_of, nout, oparent);
+  hipLaunchKernelGGL(link_parents_kernel, dim3(grid_for(nout, 256)), dim3(256), 0, s, opidx, rank_of, nout, oparent);
   CPG_HIP(hipGetLastError());
 }
 
 void launch_fill_int(int* p, int v, uint64_t n, hipStream_t s) {
   if (n == 0) return;
-  fill_int_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, v, n);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
   CPG_HIP(hipGetLastError());
 }
 
@@ -235,7 +235,7 @@ __global__ void gather_int_kernel(const int* src, const int* idx, int n, int* ou
 
 void launch_gather_int(const int* src, const int* idx, int n, int* out, hipStream_t s) {
   if (n <= 0) return;
-  gather_int_kernel<<<grid_for(static_cast<uint64_t>(n), 256), 256, 0, s>>>(src, idx, n, out);
+  hipLaunchKernelGGL(gather_int_kernel, dim3(grid_for(static_cast<uint64_t>(n), 256)), dim3(256), 0, s, src, idx, n, out);
   CPG_HIP(hipGetLastError());
 }
 }  // namespace cpg

@@ -216,7 +216,7 @@ void launch_sfxt(const SfxtArgs& args, hipStream_t stream, int* grid_out) {
   if (grid_out) *grid_out = grid;
   SfxtArgs a = args;
   void* params[] = {&a};
-  CPG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(sfxt_kernel), dim3(grid),
+  CPG_HIP(hipLaunchCooperativeKernel(sfxt_kernel, dim3(grid),
                                      dim3(kSfxtBlock), params, 0, stream));
 }
 

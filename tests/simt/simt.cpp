@@ -1,0 +1,247 @@
+// Fiber scheduler behind tests/simt/hip/hip_runtime.h (test only).
+//
+// One block runs on one OS thread as blockDim.x fibers (ucontext).  A
+// fiber runs until it reaches a collective: __syncthreads (all live
+// fibers of the block), or a wave collective (all live fibers of its
+// 64-lane wave).  When every live fiber is parked and no collective can
+// complete, the launch is reported as a deadlock and the process aborts
+// with the parked positions.  Plain launches run their blocks one after
+// another; cooperative launches run every block on its own OS thread so
+// that grid-wide barriers made of atomics can complete.
+#include "hip/hip_runtime.h"
+
+#include <ucontext.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace simt {
+namespace {
+
+enum State { kRun, kBar, kWave, kDone };
+constexpr size_t kStack = 256 * 1024;
+
+struct Block;
+struct Lane {
+  ucontext_t ctx;
+  std::vector<char> stack;
+  int tid = 0;
+  State st = kRun;
+  Block* blk = nullptr;
+};
+struct Wave {
+  int live = 0;
+  int arrived = 0;
+  unsigned gen = 0;
+  uint64_t present[2] = {0, 0};
+  uint64_t slots[2][64];
+};
+struct Block {
+  std::vector<Lane> lanes;
+  std::vector<Wave> waves;
+  int live = 0;
+  int bar_arrived = 0;
+  dim3 idx, dim, grid;
+  ucontext_t sched;
+  const std::function<void()>* body = nullptr;
+};
+
+thread_local Block* t_blk = nullptr;
+thread_local Lane* t_lane = nullptr;
+
+void lane_entry(int lo, int hi) {
+  Lane* L = reinterpret_cast<Lane*>((static_cast<uintptr_t>(static_cast<unsigned>(hi)) << 32) |
+                                    static_cast<unsigned>(lo));
+  (*L->blk->body)();
+  L->st = kDone;
+  Block* B = L->blk;
+  B->live--;
+  B->waves[L->tid / 64].live--;
+  swapcontext(&L->ctx, &B->sched);
+}
+
+void park(State s) {
+  Lane* L = t_lane;
+  L->st = s;
+  swapcontext(&L->ctx, &L->blk->sched);
+}
+
+void dump_and_abort(Block& B, const char* why) {
+  std::fprintf(stderr, "simt: %s in block %u:", why, B.idx.x);
+  for (auto& l : B.lanes)
+    if (l.st != kDone) std::fprintf(stderr, " t%d:%s", l.tid, l.st == kBar ? "bar" : l.st == kWave ? "wave" : "run");
+  std::fprintf(stderr, "\n");
+  std::abort();
+}
+
+void run_block(Block& B) {
+  const int n = static_cast<int>(B.dim.x);
+  B.lanes.resize(n);
+  B.waves.assign((n + 63) / 64, Wave{});
+  B.live = n;
+  B.bar_arrived = 0;
+  for (int t = 0; t < n; ++t) {
+    Lane& L = B.lanes[t];
+    L.tid = t;
+    L.st = kRun;
+    L.blk = &B;
+    if (L.stack.size() != kStack) L.stack.assign(kStack, 0);
+    getcontext(&L.ctx);
+    L.ctx.uc_stack.ss_sp = L.stack.data();
+    L.ctx.uc_stack.ss_size = kStack;
+    L.ctx.uc_link = nullptr;
+    const uintptr_t p = reinterpret_cast<uintptr_t>(&L);
+    makecontext(&L.ctx, reinterpret_cast<void (*)()>(lane_entry), 2,
+                static_cast<int>(p & 0xffffffffu), static_cast<int>(p >> 32));
+    B.waves[t / 64].live++;
+  }
+  t_blk = &B;
+  for (;;) {
+    bool ran = false;
+    for (auto& L : B.lanes) {
+      if (L.st != kRun) continue;
+      ran = true;
+      t_lane = &L;
+      swapcontext(&B.sched, &L.ctx);
+    }
+    if (B.live == 0) break;
+    bool released = false;
+    if (B.bar_arrived > 0 && B.bar_arrived == B.live) {
+      for (auto& L : B.lanes)
+        if (L.st == kBar) L.st = kRun;
+      B.bar_arrived = 0;
+      released = true;
+    }
+    for (size_t w = 0; w < B.waves.size(); ++w) {
+      Wave& W = B.waves[w];
+      if (W.arrived > 0 && W.arrived == W.live) {
+        for (int t = static_cast<int>(w) * 64; t < std::min(n, static_cast<int>(w) * 64 + 64); ++t)
+          if (B.lanes[t].st == kWave) B.lanes[t].st = kRun;
+        W.arrived = 0;
+        W.gen++;
+        W.present[W.gen & 1] = 0;
+        released = true;
+      }
+    }
+    if (!ran && !released) dump_and_abort(B, "deadlock (collective not reached by every live lane)");
+  }
+  t_blk = nullptr;
+  t_lane = nullptr;
+}
+
+}  // namespace
+
+dim3 thread_idx() { return dim3(static_cast<unsigned>(t_lane->tid), 0, 0); }
+dim3 block_idx() { return t_blk->idx; }
+dim3 block_dim() { return t_blk->dim; }
+dim3 grid_dim() { return t_blk->grid; }
+int lane_id() { return t_lane->tid & 63; }
+
+uint64_t* wave_sync(uint64_t v) {
+  Block& B = *t_blk;
+  Lane& L = *t_lane;
+  Wave& W = B.waves[L.tid / 64];
+  // the wave cannot mix a collective with lanes parked at a block barrier
+  for (int t = (L.tid / 64) * 64; t < std::min(static_cast<int>(B.dim.x), (L.tid / 64) * 64 + 64); ++t)
+    if (B.lanes[t].st == kBar) dump_and_abort(B, "wave collective while lanes wait at __syncthreads");
+  const unsigned g = W.gen & 1;
+  W.slots[g][L.tid & 63] = v;
+  W.present[g] |= 1ull << (L.tid & 63);
+  W.arrived++;
+  park(kWave);
+  return W.slots[g];
+}
+
+uint64_t wave_present() {
+  Block& B = *t_blk;
+  Wave& W = B.waves[t_lane->tid / 64];
+  return W.present[(W.gen & 1) ^ 1];
+}
+
+void block_sync() {
+  t_blk->bar_arrived++;
+  park(kBar);
+}
+
+void relax() { std::this_thread::yield(); }
+
+void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, bool cooperative) {
+  const unsigned nb = grid.x * grid.y * grid.z;
+  if (!cooperative) {
+    // SIMT_THREADS > 1 spreads the blocks of ordinary launches over that
+    // many OS threads, so that blocks run concurrently as they do on a GPU
+    static const int nthr = [] {
+      const char* e = std::getenv("SIMT_THREADS");
+      return e ? std::max(1, std::atoi(e)) : 1;
+    }();
+    auto worker = [&](unsigned first, unsigned step) {
+      Block B;
+      for (unsigned b = first; b < nb; b += step) {
+        B.idx = dim3(b, 0, 0);
+        B.dim = block;
+        B.grid = grid;
+        B.body = &body;
+        run_block(B);
+      }
+    };
+    if (nthr == 1 || nb == 1) {
+      worker(0, 1);
+      return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthr; ++t) th.emplace_back(worker, static_cast<unsigned>(t), static_cast<unsigned>(nthr));
+    for (auto& t : th) t.join();
+    return;
+  }
+  std::vector<std::thread> th;
+  std::vector<Block> blocks(nb);
+  for (unsigned b = 0; b < nb; ++b) {
+    blocks[b].idx = dim3(b, 0, 0);
+    blocks[b].dim = block;
+    blocks[b].grid = grid;
+    blocks[b].body = &body;
+    th.emplace_back([&blocks, b]() { run_block(blocks[b]); });
+  }
+  for (auto& t : th) t.join();
+}
+
+}  // namespace simt
+
+// ---------------------------------------------------------------- host API
+struct simt_event { std::chrono::steady_clock::time_point t; };
+struct simt_stream { int dummy; };
+
+const char* hipGetErrorString(hipError_t) { return "simt"; }
+hipError_t hipGetLastError() { return hipSuccess; }
+hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
+  *v = a == hipDeviceAttributeMultiprocessorCount ? 2 : 1;
+  return hipSuccess;
+}
+hipError_t simt_malloc(void** p, size_t bytes) {
+  *p = std::calloc(1, bytes ? bytes : 1);
+  return *p ? hipSuccess : 2;
+}
+hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
+hipError_t hipMemsetAsync(void* p, int v, size_t n, hipStream_t) { std::memset(p, v, n); return hipSuccess; }
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = new simt_stream{0}; return hipSuccess; }
+hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t* e) { *e = new simt_event{}; return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = std::chrono::steady_clock::now(); return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+  *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
+  return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
