@@ -220,6 +220,7 @@ struct cpg_handle {
     CPG_HIP(hipStreamSynchronize(stream));
     if (kernel_timing && n > 0) timing.sfxt_kernel_ms = ev_sfxt.ms();
     if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
+    if (c.nsv_err) throw std::runtime_error("SFXT next-smaller pointer walk exceeded n steps");
     diameter = c.num_levels;
     uint32_t bits = c.max_abs_distf_bits;
     std::memcpy(&max_abs_distf, &bits, 4);
@@ -327,9 +328,12 @@ struct cpg_handle {
       wa.nodes = ns; wa.hist = hist_all.p; wa.ctl = pctl.p;
       wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = g.max_abs_wgt;
       wa.collect_stats = o.collect_stats;
+      wa.step_limit = static_cast<int>(std::min<int64_t>(
+          0x3fffffff, (static_cast<int64_t>(diameter) + 2) * (g.max_out_degree + 2) + 16));
       bool overflow = false;
       for (int r = 0;; ++r) {
-        if (r > 1000000) throw std::runtime_error("PFXT round limit exceeded");
+        if (r > 100000) throw std::runtime_error("PFXT round limit exceeded");
+        if (now_ms() - tq0 > 120000.0) throw std::runtime_error("PFXT query exceeded 120 s");
         LiveArgs l{live_key[cur].p, live_idx[cur].p, live_key[cur ^ 1].p, live_idx[cur ^ 1].p,
                    batch.p, hist_all.p, hist_pend.p, pctl.p};
         launch_round_begin(pctl.p, stream);
@@ -356,6 +360,7 @@ struct cpg_handle {
           timing.walk_fill_ms += ev_fill.ms();
           timing.walk_launches += 1;
         }
+        if (c.err) throw std::runtime_error("PFXT walk exceeded its step bound");
         if (c.overflow) { overflow = true; break; }
         if (c.done) break;  // the round did nothing: live stays in `cur`
         cur ^= 1;

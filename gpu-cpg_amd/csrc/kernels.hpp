@@ -16,7 +16,7 @@ struct SfxtCtl {
   int num_levels;               // graph diameter (levels peeled)
   unsigned levelized;           // vertices peeled
   unsigned max_abs_distf_bits;  // max |distf| as float bits
-  unsigned pad;
+  int nsv_err;                  // a next-smaller walk exceeded n steps
 };
 
 struct SfxtArgs {
@@ -62,6 +62,8 @@ struct PfxtCtl {
   int overflow;                    // node capacity exceeded in a fill pass
   int rounds;
   int swap_pending;                // compaction output becomes the live buffer
+  int err;                         // a walk exceeded its step bound (corrupt index)
+  int pad;
 };
 
 struct NodeStore {
@@ -86,6 +88,7 @@ struct WalkArgs {
   float max_abs_distf;
   float max_abs_wgt;
   int collect_stats;
+  int step_limit;         // bound on one walk's steps: (levels + 2) * (max out-degree + 2)
 };
 
 struct LiveArgs {

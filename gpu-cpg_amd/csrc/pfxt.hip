@@ -196,10 +196,17 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
     }
     int j = 0, je = 0, sv = -1;
     float dv = 0.0f;
+    int steps = 0;
     while (__ballot(mode != 2)) {
       bool emit = false;
       float cc = 0.0f;
       int cw = 0;
+      // a walk visits each chain vertex and each of its deviations at most
+      // once; more steps than that means a corrupt index, so stop and flag
+      if (mode != 2 && ++steps > a.step_limit) {
+        atomicExch(&ctl->err, 1);
+        mode = 2;
+      }
       if (mode == 0) {
         const int4 r = a.vrec[v];
         ++hops;

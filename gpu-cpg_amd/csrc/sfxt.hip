@@ -26,7 +26,7 @@ namespace {
 
 constexpr int kSfxtBlock = 256;
 constexpr int kLdsQueue = 2048;
-constexpr unsigned kSpinLimit = 1u << 25;
+constexpr unsigned kSpinLimit = 1u << 22;
 
 __device__ inline unsigned atomic_load_rlx(unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -119,7 +119,12 @@ __device__ void settle_vertex(const SfxtArgs& a, int u, int d, float* local_max)
   }
   const float mind = cnt ? __int_as_float(a.devs[b].w) : __builtin_huge_valf();
   int x = s;
-  while (x != -1) {
+  for (int guard = 0; x != -1; ++guard) {
+    if (guard > a.n) {  // pointers only move toward the sinks; never taken
+      atomicExch(&a.ctl->nsv_err, 1);
+      x = -1;
+      break;
+    }
     const int4 r = a.vrec[x];
     if (__int_as_float(r.x) < mind) break;
     x = r.y;
