@@ -91,18 +91,22 @@ struct cpg_handle {
   int grid = 0;  // grid-stride kernels
 
   // device graph
-  DevBuf<int> oadjp, oadj, iadjp, iadj, srcs;
+  DevBuf<int> oadjp, oadj, iadjp, srcs, sinks;
+  DevBuf<int2> iedge;
   DevBuf<float> owgt;
   bool graph_on_device = false;
 
   // SFXT
-  DevBuf<int> deps, queue, lvlp, level, dist, succ, src_dist;
+  DevBuf<int> deps, queue, lvlp, level, dist, succ, src_dist, big;
+  DevBuf<unsigned long long> best;
   DevBuf<float> distf;
-  DevBuf<int4> vrec, devs;
-  DevBuf<int2> devinfo;
+  DevBuf<int4> vrec, devs, devs_tmp;
+  DevBuf<int2> devinfo, lift;
   DevBuf<SfxtCtl> sctl;
   bool sfxt_valid = false;
   int diameter = 0;
+  int sfxt_blocks = 0;       // level-loop grid requested (0 = automatic)
+  int last_diameter = 0;     // levels of the last SFXT on this graph (automatic grid)
   float max_abs_distf = 0.0f;
   bool cache_enabled = false;
   int cache_hits = 0, cache_misses = 0;
@@ -132,7 +136,7 @@ struct cpg_handle {
 
   cpg_timing timing{};
   bool kernel_timing = false;
-  EventPair ev_sfxt, ev_count, ev_fill;
+  EventPair ev_sfxt, ev_levels, ev_count, ev_fill;
 
   cpg_handle() {}
   ~cpg_handle() {
@@ -154,6 +158,7 @@ struct cpg_handle {
     g = std::move(ng);
     loaded = true;
     graph_on_device = false;
+    last_diameter = 0;
     invalidate_cache_payload();
     cache_hits = cache_misses = 0;  // read_input clears the static cache (gpucpg.cu:454)
     reset();
@@ -177,16 +182,37 @@ struct cpg_handle {
     ensure_stream();
     const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
     oadjp.reserve(n + 1); oadj.reserve(m); owgt.reserve(m);
-    iadjp.reserve(n + 1); iadj.reserve(m); srcs.reserve(g.srcs.size());
+    iadjp.reserve(n + 1); iedge.reserve(m); srcs.reserve(g.srcs.size());
+    sinks.reserve(g.sinks.size());
     h2d(oadjp.p, g.oadjp.data(), n + 1, stream);
     h2d(oadj.p, g.oadj.data(), m, stream);
     h2d(owgt.p, g.owgt.data(), m, stream);
     h2d(iadjp.p, g.iadjp.data(), n + 1, stream);
-    h2d(iadj.p, g.iadj.data(), m, stream);
+    {
+      std::vector<int2> ie(m);
+      for (size_t j = 0; j < m; ++j) {
+        int wb;
+        std::memcpy(&wb, &g.iwgt[j], 4);
+        ie[j] = make_int2(g.iadj[j], wb);
+      }
+      h2d(iedge.p, ie.data(), m, stream);
+      CPG_HIP(hipStreamSynchronize(stream));  // ie goes out of scope
+    }
     h2d(srcs.p, g.srcs.data(), g.srcs.size(), stream);
+    h2d(sinks.p, g.sinks.data(), g.sinks.size(), stream);
     CPG_HIP(hipStreamSynchronize(stream));
     graph_on_device = true;
     sfxt_valid = false;
+  }
+
+  // Level-loop grid: about one thread per vertex of an average level (known
+  // from the last levelization of this graph), within [16, 64] blocks; a
+  // grid barrier per level costs more with more blocks.
+  int level_loop_blocks() const {
+    if (sfxt_blocks > 0) return sfxt_blocks;
+    if (last_diameter <= 0) return 64;
+    const int64_t per_level = g.n / last_diameter;
+    return static_cast<int>(std::min<int64_t>(64, std::max<int64_t>(16, (per_level + 255) / 256)));
   }
 
   // Runs the SFXT unless a cached one is valid.  Returns elapsed ms.
@@ -201,27 +227,49 @@ struct cpg_handle {
     const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
     deps.reserve(n); queue.reserve(n); lvlp.reserve(n + 2); level.reserve(n);
     dist.reserve(n); succ.reserve(n); distf.reserve(n); vrec.reserve(n);
-    devinfo.reserve(n); devs.reserve(m); sctl.reserve(1);
+    devinfo.reserve(n); devs.reserve(m); big.reserve(n); best.reserve(n); sctl.reserve(1);
+    if (g.max_out_degree > kSfxtLdsSort) devs_tmp.reserve(m);
     CPG_HIP(hipMemsetAsync(sctl.p, 0, sizeof(SfxtCtl), stream));
+    SfxtCtl c{};
     if (n > 0) {
       SfxtArgs a{};
       a.n = g.n;
+      a.nsinks = static_cast<int>(g.sinks.size());
       a.oadjp = oadjp.p; a.oadj = oadj.p; a.owgt = owgt.p;
-      a.iadjp = iadjp.p; a.iadj = iadj.p;
+      a.iadjp = iadjp.p; a.iedge = iedge.p; a.sinks = sinks.p; a.best = best.p;
       a.deps = deps.p; a.queue = queue.p; a.lvlp = lvlp.p; a.level = level.p;
       a.dist = dist.p; a.succ = succ.p; a.distf = distf.p;
-      a.vrec = vrec.p; a.devinfo = devinfo.p; a.devs = devs.p; a.ctl = sctl.p;
-      if (kernel_timing) { ev_sfxt.create(); CPG_HIP(hipEventRecord(ev_sfxt.a, stream)); }
-      launch_sfxt(a, stream, nullptr);
+      a.vrec = vrec.p; a.devinfo = devinfo.p; a.devs = devs.p; a.big = big.p;
+      a.devs_tmp = g.max_out_degree > kSfxtLdsSort ? devs_tmp.p : nullptr;
+      a.ctl = sctl.p;
+      if (kernel_timing) {
+        ev_sfxt.create();
+        ev_levels.create();
+        CPG_HIP(hipEventRecord(ev_sfxt.a, stream));
+        CPG_HIP(hipEventRecord(ev_levels.a, stream));
+      }
+      timing.sfxt_blocks = launch_sfxt_levels(a, level_loop_blocks(), stream);
+      if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.b, stream));
+      d2h(&c, sctl.p, 1, stream);
+      CPG_HIP(hipStreamSynchronize(stream));
+      if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
+      const int lv = sfxt_lift_levels(c.num_levels);
+      lift.reserve(static_cast<size_t>(lv) * n);
+      a.lift = lift.p;
+      launch_sfxt_index(a, lv, stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_sfxt.b, stream));
     }
-    SfxtCtl c{};
     d2h(&c, sctl.p, 1, stream);
     CPG_HIP(hipStreamSynchronize(stream));
-    if (kernel_timing && n > 0) timing.sfxt_kernel_ms = ev_sfxt.ms();
+    if (kernel_timing && n > 0) {
+      timing.sfxt_kernel_ms = ev_sfxt.ms();
+      timing.sfxt_levels_ms = ev_levels.ms();
+    }
     if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
-    if (c.nsv_err) throw std::runtime_error("SFXT next-smaller pointer walk exceeded n steps");
+    if (c.nsv_err == 2) throw std::runtime_error("SFXT long deviation list without sort buffer");
+    if (c.nsv_err) throw std::runtime_error("SFXT next-smaller descent found no answer");
     diameter = c.num_levels;
+    last_diameter = diameter;
     uint32_t bits = c.max_abs_distf_bits;
     std::memcpy(&max_abs_distf, &bits, 4);
     sfxt_valid = true;
@@ -248,6 +296,8 @@ struct cpg_handle {
     nres = 0;
     timing = cpg_timing{};
     kernel_timing = o.kernel_timing != 0;
+    if (o.sfxt_blocks < 0) throw std::runtime_error("sfxt_blocks must be >= 0");
+    sfxt_blocks = o.sfxt_blocks;
     const double tq0 = now_ms();
     timing.sfxt_ms = run_sfxt(true);
     if (k <= 0 || g.n == 0) {

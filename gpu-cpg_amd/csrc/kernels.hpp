@@ -16,16 +16,21 @@ struct SfxtCtl {
   int num_levels;               // graph diameter (levels peeled)
   unsigned levelized;           // vertices peeled
   unsigned max_abs_distf_bits;  // max |distf| as float bits
-  int nsv_err;                  // a next-smaller walk exceeded n steps
+  int nsv_err;                  // 1: nsv descent found no answer, 2: long list without devs_tmp
+  unsigned nbig;                // deviation lists longer than one thread sorts
+  unsigned pad;
 };
 
 struct SfxtArgs {
   int n;
+  int nsinks;
   const int* oadjp;
   const int* oadj;
   const float* owgt;
   const int* iadjp;
-  const int* iadj;
+  const int2* iedge;  // fanin records {x, w_e bits}, in-CSR order
+  const int* sinks;   // out-degree-0 vertices: level 0
+  unsigned long long* best;  // packed relax candidate per vertex (see sfxt.hip)
   int* deps;
   int* queue;
   int* lvlp;
@@ -36,10 +41,19 @@ struct SfxtArgs {
   int4* vrec;      // {min delta bits, nsv, succ, distf bits}
   int2* devinfo;   // {first deviation slot, deviation count}
   int4* devs;      // {w, w_e bits, distf[w] bits, delta bits}, m slots
+  int4* devs_tmp;  // m slots, only when some fanout exceeds the LDS sort (else null)
+  int* big;        // vertices whose deviation lists are sorted block-wide, n slots
+  int2* lift;      // lift levels x n: {2^k-th successor, min delta over those 2^k}
   SfxtCtl* ctl;
 };
 
-void launch_sfxt(const SfxtArgs& args, hipStream_t stream, int* grid_out);
+// Number of lift levels for a graph of num_levels levels (2^levels > chain length + 1).
+int sfxt_lift_levels(int num_levels);
+// Init + the cooperative level loop; returns the grid used.
+int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);
+// Deviation records, their sort, the lift tables and the nsv pointers.
+void launch_sfxt_index(const SfxtArgs& args, int lift_levels, hipStream_t stream);
+constexpr int kSfxtLdsSort = 2048;  // longest deviation list sorted in LDS
 
 // ---------------- PFXT ----------------
 struct PfxtCtl {

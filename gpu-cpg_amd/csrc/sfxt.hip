@@ -1,5 +1,5 @@
-// SFXT: levelize + relax + successor + distf + deviation index, one
-// cooperative persistent kernel with a grid barrier between levels.
+// SFXT: levelize + relax + successor + distf, then the deviation index the
+// PFXT walk reads.
 //
 // Semantics follow the reference path:
 //   levelize    bfs_td_step_privatized   gpucpg/gpucpg.cu:2215-2283, loop :17642-17681
@@ -8,15 +8,28 @@
 //               dist[v] = min_e dist[w] + (int)(w_e * 10000.0f), sinks 0 (:17035-17039)
 //   successor   update_successors_bu     gpucpg/gpucpg.cu:2606-2628 (largest id at the min)
 //   distf       (float)dist / SCALE_UP, correctly rounded (used at :2672-2673)
-// Because every fanout of a level-d vertex sits on a lower level, relax and
-// successor for level d run in the same pass that peels level d.
+// Every fanout of a level-d vertex sits on a lower level, so relax and
+// successor for level d run in the same pass that peels level d.  That
+// pass is the only part with a level-to-level dependency; it runs as one
+// cooperative kernel on a small grid (the per-level grid barrier costs more
+// with more blocks, and levels hold a few thousand vertices) and does no
+// more than the relax, the successor and the fanin release.
 //
-// The deviation index is this engine's own structure for the PFXT walk:
-// per vertex u, its deviation edges (u->w, w != succ[u], dist[w] finite:
-// gpucpg/gpucpg.cu:2662-2668) sorted by delta = distf[w] + w_e - distf[u],
-// the smallest delta, and nsv[u] = the first vertex after u on u's suffix
-// chain whose smallest delta is strictly smaller (next-smaller-value
-// pointer), so a walk can skip chain segments with no cheap deviation.
+// The deviation index is this engine's own structure for the PFXT walk,
+// built afterwards by fully parallel passes:
+//   devs     per vertex u, its deviation edges (u->w, w != succ[u], dist[w]
+//            finite: gpucpg/gpucpg.cu:2662-2668) as {w, w_e, distf[w], delta}
+//            with delta = distf[w] + w_e - distf[u], sorted by delta (stable)
+//   vrec     {smallest delta, nsv, succ, distf}; nsv[u] is the first vertex
+//            after u on u's suffix chain whose smallest delta is strictly
+//            smaller (next-smaller-value pointer), so a walk can skip chain
+//            segments with no cheap deviation
+// nsv comes from binary lifting over the successor forest: lift[k][u] =
+// {2^k-th successor of u, smallest delta over the 2^k chain vertices after
+// u}; a greedy descent from the top level finds the first chain vertex
+// whose smallest delta is below u's with one dependent load per level.
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -27,6 +40,9 @@ namespace {
 constexpr int kSfxtBlock = 256;
 constexpr int kLdsQueue = 2048;
 constexpr unsigned kSpinLimit = 1u << 22;
+constexpr int kThreadSort = 32;    // deviation lists up to this length: one thread sorts
+constexpr int kLdsSort = kSfxtLdsSort;  // up to this length: one block sorts in LDS
+constexpr int kSortBlock = 256;
 
 __device__ inline unsigned atomic_load_rlx(unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -75,67 +91,55 @@ __device__ inline float wave_max(float v) {
   return v;
 }
 
-// Relax, successor, distf and deviation index of one vertex of level d.
-__device__ void settle_vertex(const SfxtArgs& a, int u, int d, float* local_max) {
-  const int b = a.oadjp[u], e = a.oadjp[u + 1];
-  int best = d == 0 ? 0 : kIntMax;
-  int s = -1;
-  if (d > 0) {
-    for (int j = b; j < e; ++j) {
-      const int w = a.oadj[j];
-      const int wi = static_cast<int>(a.owgt[j] * static_cast<float>(kScaleUp));
-      const int cand = a.dist[w] + wi;
-      if (cand < best) { best = cand; s = w; }
-      else if (cand == best && w > s) s = w;
-    }
-  }
-  const float df = static_cast<float>(best) / static_cast<float>(kScaleUp);
-  a.dist[u] = best;
-  a.succ[u] = s;
-  a.level[u] = d;
-  a.distf[u] = df;
-  *local_max = fmaxf(*local_max, fabsf(df));
-  // deviation records {w, w_e bits, distf[w] bits, delta bits}, sorted by delta
-  int cnt = 0;
-  for (int j = b; j < e; ++j) {
-    const int w = a.oadj[j];
-    if (w == s) continue;
-    const int dw = a.dist[w];
-    if (dw == kIntMax) continue;
-    const float fw = a.distf[w];
-    const float we = a.owgt[j];
-    const float delta = static_cast<float>(static_cast<double>(fw) + static_cast<double>(we) -
-                                           static_cast<double>(df));
-    int4 rec = make_int4(w, __float_as_int(we), __float_as_int(fw), __float_as_int(delta));
-    int k = cnt;
-    while (k > 0) {
-      const int4 prev = a.devs[b + k - 1];
-      if (__int_as_float(prev.w) <= delta) break;
-      a.devs[b + k] = prev;
-      --k;
-    }
-    a.devs[b + k] = rec;
-    ++cnt;
-  }
-  const float mind = cnt ? __int_as_float(a.devs[b].w) : __builtin_huge_valf();
-  int x = s;
-  for (int guard = 0; x != -1; ++guard) {
-    if (guard > a.n) {  // pointers only move toward the sinks; never taken
-      atomicExch(&a.ctl->nsv_err, 1);
-      x = -1;
-      break;
-    }
-    const int4 r = a.vrec[x];
-    if (__int_as_float(r.x) < mind) break;
-    x = r.y;
-  }
-  a.vrec[u] = make_int4(__float_as_int(mind), x, s, __float_as_int(df));
-  a.devinfo[u] = make_int2(b, cnt);
+// Candidate key: high word = candidate distance with the sign bit flipped
+// (unsigned order = signed order), low word = 0x7fffffff - fanout id (a
+// larger id orders first among equal distances).
+__device__ inline unsigned long long pack_cand(int dist, int w) {
+  return (static_cast<unsigned long long>(static_cast<uint32_t>(dist) ^ 0x80000000u) << 32) |
+         static_cast<uint32_t>(0x7fffffff - w);
+}
+__device__ inline int unpack_cand(unsigned long long k) {
+  return static_cast<int>(static_cast<uint32_t>(k >> 32) ^ 0x80000000u);
+}
+__device__ inline int unpack_succ(unsigned long long k) {
+  return 0x7fffffff - static_cast<int>(static_cast<uint32_t>(k));
+}
+
+__device__ inline int4 default_vrec() {
+  return make_int4(__float_as_int(__builtin_huge_valf()), -1, -1, 0);
 }
 
 }  // namespace
 
-__global__ void __launch_bounds__(kSfxtBlock) sfxt_kernel(SfxtArgs a) {
+// Per-vertex state before the level loop; the level-0 queue is the sink list.
+__global__ void __launch_bounds__(256) sfxt_init_kernel(SfxtArgs a) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gthreads = gridDim.x * blockDim.x;
+  for (int v = gtid; v < a.n; v += gthreads) {
+    a.deps[v] = a.oadjp[v + 1] - a.oadjp[v];
+    a.level[v] = -1;
+    a.dist[v] = kIntMax;
+    a.succ[v] = -1;
+    a.distf[v] = static_cast<float>(kIntMax) / static_cast<float>(kScaleUp);
+    a.best[v] = ~0ull;
+  }
+  for (int i = gtid; i < a.nsinks; i += gthreads) a.queue[i] = a.sinks[i];
+  if (gtid == 0) a.ctl->qtail = static_cast<unsigned>(a.nsinks);
+}
+
+// Level loop: peels level d (the vertices whose fanouts are all settled)
+// and releases their fanins; a grid barrier separates levels.
+//
+// The relax runs push-style: a settled vertex w offers every fanin x the
+// candidate dist[w] + (int)(w_e * 10000.0f) through a 64-bit atomic min on
+// best[x], whose high word orders the candidate and whose low word orders
+// larger w first.  When x is peeled, best[x] therefore holds exactly the
+// minimum candidate and, among equal minima, the largest fanout id: the
+// relax and successor rules above, read with one load instead of a gather
+// over x's fanouts.  Per vertex the level loop makes four dependent memory
+// round trips (queue entry; best and fanin range; fanin records; the
+// atomics), with all loads of a round issued together.
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
   __shared__ int s_q[kLdsQueue];
   __shared__ int s_qn;
   __shared__ unsigned s_base;
@@ -161,33 +165,45 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_kernel(SfxtArgs a) {
     if (threadIdx.x == 0) s_qn = 0;
   };
 
-  // phase 0: per-vertex init; sinks (out-degree 0) form level 0
-  for (int v = gtid; v < a.n; v += gthreads) {
-    const int deg = a.oadjp[v + 1] - a.oadjp[v];
-    a.deps[v] = deg;
-    a.level[v] = -1;
-    a.dist[v] = kIntMax;
-    a.succ[v] = -1;
-    a.distf[v] = static_cast<float>(kIntMax) / static_cast<float>(kScaleUp);
-    a.vrec[v] = make_int4(__float_as_int(__builtin_huge_valf()), -1, -1, 0);
-    a.devinfo[v] = make_int2(a.oadjp[v], 0);
-    if (deg == 0) push(v);
-  }
-  flush();
-  unsigned le = 0;
-  if (!grid_barrier(ctl, nblocks, &le)) return;
-
-  unsigned lb = 0;
+  unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
   int d = 0;
   float local_max = 0.0f;
   while (true) {
     if (gtid == 0) a.lvlp[d] = static_cast<int>(lb);
     for (unsigned i = lb + gtid; i < le; i += gthreads) {
       const int u = a.queue[i];
-      settle_vertex(a, u, d, &local_max);
-      for (int j = a.iadjp[u]; j < a.iadjp[u + 1]; ++j) {
-        const int x = a.iadj[j];
-        if (atomicSub(&a.deps[x], 1) == 1) push(x);
+      const unsigned long long key = a.best[u];
+      const int b = a.iadjp[u], e = a.iadjp[u + 1];
+      int du = 0, s = -1;
+      if (d > 0) {
+        du = unpack_cand(key);
+        s = unpack_succ(key);
+      }
+      const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
+      a.dist[u] = du;
+      a.succ[u] = s;
+      a.level[u] = d;
+      a.distf[u] = df;
+      local_max = fmaxf(local_max, fabsf(df));
+      for (int j = b; j < e; j += 4) {
+        // four fanin records; past the end, repeat the last one with a
+        // no-op min and a zero decrement, so every load is unconditional
+        int2 r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = a.iedge[min(j + q, e - 1)];
+        unsigned long long cand[4];
+        int old[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const bool ok = j + q < e;
+          const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
+          cand[q] = ok ? pack_cand(du + wi, u) : ~0ull;
+          atomicMin(&a.best[r[q].x], cand[q]);
+          old[q] = atomicSub(&a.deps[r[q].x], ok ? 1 : 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (j + q < e && old[q] == 1) push(r[q].x);
       }
     }
     flush();
@@ -207,22 +223,208 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_kernel(SfxtArgs a) {
   }
 }
 
-void launch_sfxt(const SfxtArgs& args, hipStream_t stream, int* grid_out) {
-  int dev = 0;
+// Deviation records of every settled vertex in fanout order, their smallest
+// delta, and the vertex record without its nsv pointer.  Lists of up to
+// kThreadSort records are sorted here (stable insertion sort by delta);
+// longer ones are queued for sfxt_sort_big_kernel.
+__global__ void __launch_bounds__(256) sfxt_devs_kernel(SfxtArgs a) {
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
+    const int b = a.oadjp[u], e = a.oadjp[u + 1];
+    if (a.level[u] < 0) {
+      a.vrec[u] = default_vrec();
+      a.devinfo[u] = make_int2(b, 0);
+      continue;
+    }
+    const int s = a.succ[u];
+    const float df = a.distf[u];
+    int cnt = 0;
+    float mind = __builtin_huge_valf();
+    for (int j = b; j < e; ++j) {
+      const int w = a.oadj[j];
+      if (w == s) continue;
+      if (a.dist[w] == kIntMax) continue;
+      const float fw = a.distf[w];
+      const float we = a.owgt[j];
+      const float delta = static_cast<float>(static_cast<double>(fw) + static_cast<double>(we) -
+                                             static_cast<double>(df));
+      a.devs[b + cnt] = make_int4(w, __float_as_int(we), __float_as_int(fw), __float_as_int(delta));
+      mind = fminf(mind, delta);
+      ++cnt;
+    }
+    if (cnt <= kThreadSort) {
+      for (int i = 1; i < cnt; ++i) {
+        const int4 rec = a.devs[b + i];
+        const float key = __int_as_float(rec.w);
+        int k = i;
+        while (k > 0) {
+          const int4 prev = a.devs[b + k - 1];
+          if (__int_as_float(prev.w) <= key) break;
+          a.devs[b + k] = prev;
+          --k;
+        }
+        if (k != i) a.devs[b + k] = rec;
+      }
+    } else {
+      a.big[atomicAdd(&a.ctl->nbig, 1u)] = u;
+    }
+    a.vrec[u] = make_int4(__float_as_int(mind), -1, s, __float_as_int(df));
+    a.devinfo[u] = make_int2(b, cnt);
+  }
+}
+
+// Stable sort by delta of the long deviation lists, one block per list.
+// A record's rank is the number of records with a smaller delta plus the
+// number of earlier records with an equal delta, which reproduces a stable
+// insertion sort.  Lists of up to kLdsSort records are ranked in LDS; longer
+// ones (only when the graph has such fanouts) rank from global memory and
+// permute through devs_tmp, the same slot range of a second m-record buffer.
+__global__ void __launch_bounds__(kSortBlock) sfxt_sort_big_kernel(SfxtArgs a) {
+  __shared__ int4 s_rec[kLdsSort];
+  const unsigned nbig = a.ctl->nbig;
+  for (unsigned t = blockIdx.x; t < nbig; t += gridDim.x) {
+    const int u = a.big[t];
+    const int2 di = a.devinfo[u];
+    const int b = di.x, cnt = di.y;
+    if (cnt <= kLdsSort) {
+      for (int i = threadIdx.x; i < cnt; i += blockDim.x) s_rec[i] = a.devs[b + i];
+      __syncthreads();
+      for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const float key = __int_as_float(s_rec[i].w);
+        int rank = 0;
+        for (int j = 0; j < cnt; ++j) {
+          const float kj = __int_as_float(s_rec[j].w);
+          rank += (kj < key || (kj == key && j < i)) ? 1 : 0;
+        }
+        a.devs[b + rank] = s_rec[i];
+      }
+      __syncthreads();
+    } else if (a.devs_tmp) {
+      for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int4 rec = a.devs[b + i];
+        const float key = __int_as_float(rec.w);
+        int rank = 0;
+        for (int j = 0; j < cnt; ++j) {
+          const float kj = __int_as_float(a.devs[b + j].w);
+          rank += (kj < key || (kj == key && j < i)) ? 1 : 0;
+        }
+        a.devs_tmp[b + rank] = rec;
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < cnt; i += blockDim.x) a.devs[b + i] = a.devs_tmp[b + i];
+      __syncthreads();
+    } else if (threadIdx.x == 0) {
+      atomicExch(&a.ctl->nsv_err, 2);  // host did not provide devs_tmp
+    }
+  }
+}
+
+// lift[0][u] = {succ[u], smallest delta of succ[u]} ({-1, +inf} at a chain end).
+__global__ void __launch_bounds__(256) sfxt_lift0_kernel(SfxtArgs a) {
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
+    const int s = a.succ[u];
+    a.lift[u] = s < 0 ? make_int2(-1, __float_as_int(__builtin_huge_valf()))
+                      : make_int2(s, a.vrec[s].x);
+  }
+}
+
+// lift[k][u] from lift[k-1]: jump twice, minimum over both segments.
+__global__ void __launch_bounds__(256) sfxt_liftk_kernel(SfxtArgs a, int k) {
+  const size_t n = static_cast<size_t>(a.n);
+  const int2* prev = a.lift + static_cast<size_t>(k - 1) * n;
+  int2* cur = a.lift + static_cast<size_t>(k) * n;
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
+    const int2 p = prev[u];
+    if (p.x < 0) {
+      cur[u] = p;
+    } else {
+      const int2 q = prev[p.x];
+      cur[u] = make_int2(q.x, __float_as_int(fminf(__int_as_float(p.y), __int_as_float(q.y))));
+    }
+  }
+}
+
+// nsv[u]: greedy descent over the lift levels.  At level k the walk jumps
+// 2^k chain vertices when none of them has a smallest delta strictly below
+// t = u's; the chain vertex after the final position is the answer.  With
+// 2^levels above the longest chain, the descent lands on the vertex just
+// before the first qualifying one, or runs off the chain end (no answer).
+__global__ void __launch_bounds__(256) sfxt_nsv_kernel(SfxtArgs a, int levels) {
+  const size_t n = static_cast<size_t>(a.n);
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
+    int4 r = a.vrec[u];
+    if (r.z < 0) continue;  // chain end or unsettled: nsv stays -1
+    const float t = __int_as_float(r.x);
+    int x = u;
+    for (int k = levels - 1; k >= 0 && x >= 0; --k) {
+      const int2 e = a.lift[static_cast<size_t>(k) * n + x];
+      if (!(__int_as_float(e.y) < t)) x = e.x;  // skip 2^k vertices (or fall off the end)
+    }
+    int nsv = -1;
+    if (x >= 0) {
+      const int2 e0 = a.lift[x];
+      if (e0.x >= 0 && __int_as_float(e0.y) < t) nsv = e0.x;
+      else atomicExch(&a.ctl->nsv_err, 1);  // cannot happen with enough levels
+    }
+    r.y = nsv;
+    a.vrec[u] = r;
+  }
+}
+
+namespace {
+
+int device_cus() {
+  int dev = 0, cus = 0;
   CPG_HIP(hipGetDevice(&dev));
-  int cus = 0, coop = 0;
   CPG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus;
+}
+
+int flat_grid(int n, int cus) {
+  return std::max(1, std::min((n + 255) / 256, cus * 8));
+}
+
+}  // namespace
+
+int sfxt_lift_levels(int num_levels) {
+  int k = 1;
+  while ((1 << k) < num_levels + 1 && k < 30) ++k;
+  return k;
+}
+
+int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
+  int dev = 0, coop = 0;
+  CPG_HIP(hipGetDevice(&dev));
   CPG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
   if (!coop) throw std::runtime_error("device does not support cooperative launch");
+  const int cus = device_cus();
   int per_cu = 0;
-  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_kernel, kSfxtBlock, 0));
+  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_levels_kernel, kSfxtBlock, 0));
   if (per_cu < 1) throw std::runtime_error("sfxt kernel cannot be resident");
-  const int grid = cus;  // one block per CU: fewer barrier arrivals and fences
-  if (grid_out) *grid_out = grid;
+  const int grid = std::max(1, std::min(blocks, cus * per_cu));
+  hipLaunchKernelGGL(sfxt_init_kernel, dim3(flat_grid(args.n, cus)), dim3(256), 0, stream, args);
+  CPG_HIP(hipGetLastError());
   SfxtArgs a = args;
   void* params[] = {&a};
-  CPG_HIP(hipLaunchCooperativeKernel(sfxt_kernel, dim3(grid),
-                                     dim3(kSfxtBlock), params, 0, stream));
+  CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_kernel, dim3(grid), dim3(kSfxtBlock), params, 0,
+                                     stream));
+  return grid;
+}
+
+void launch_sfxt_index(const SfxtArgs& args, int lift_levels, hipStream_t stream) {
+  const int cus = device_cus();
+  const int g = flat_grid(args.n, cus);
+  hipLaunchKernelGGL(sfxt_devs_kernel, dim3(g), dim3(256), 0, stream, args);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sfxt_sort_big_kernel, dim3(cus * 2), dim3(kSortBlock), 0, stream, args);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sfxt_lift0_kernel, dim3(g), dim3(256), 0, stream, args);
+  CPG_HIP(hipGetLastError());
+  for (int k = 1; k < lift_levels; ++k) {
+    hipLaunchKernelGGL(sfxt_liftk_kernel, dim3(g), dim3(256), 0, stream, args, k);
+    CPG_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(sfxt_nsv_kernel, dim3(g), dim3(256), 0, stream, args, lift_levels);
+  CPG_HIP(hipGetLastError());
 }
 
 }  // namespace cpg

@@ -52,6 +52,7 @@ typedef struct {
   int collect_stats;        /* 1: count hops / deviations / emitted children  */
   int kernel_timing;        /* 1: time the SFXT and walk kernels with HIP events
                                on the engine's stream (cpg_timing.*_kernel_ms) */
+  int sfxt_blocks;          /* SFXT level-loop grid in blocks (0 = automatic) */
 } cpg_query_opts;
 
 /* Timings of the last query, milliseconds (gpucpg/gpucpg.cuh:327-356). */
@@ -72,6 +73,8 @@ typedef struct {
   double walk_count_ms;   /* kernel_timing: count-pass walk kernels, summed              */
   double walk_fill_ms;    /* kernel_timing: fill-pass walk kernels, summed               */
   int64_t walk_launches;  /* kernel_timing: walk kernel launches of each kind            */
+  double sfxt_levels_ms;  /* kernel_timing: SFXT init + level loop (part of sfxt_kernel_ms)  */
+  int64_t sfxt_blocks;    /* SFXT level-loop grid used                                    */
 } cpg_timing;
 
 /* ---- lifetime ---- */

@@ -117,3 +117,55 @@ def test_c2_topk_prefixes_bit_exact():
     say(f"C2 timings {g.timings()}")
     for kp in (1000, 10_000, 50_000, 100_000, 1_000_000):
         assert_bits_equal(want[:kp], got[:kp], f"C2 prefix {kp}")
+
+
+def wide_fanout_graph():
+    """Two wide vertices: one with 3000 fanouts (longer than the 2048-record
+    LDS sort, so its deviation list is sorted through the second buffer) and
+    one with 40 (sorted in LDS), plus tied weights for the stable order."""
+    rng = np.random.default_rng(7)
+    f1, f2, ns, mid0 = 3000, 40, 3, 2
+    n = mid0 + f1 + ns
+    u, v, w = [], [], []
+    for i in range(f1):
+        u.append(0); v.append(mid0 + i); w.append(rng.uniform(0.01, 1))
+    for i in range(f2):
+        u.append(1); v.append(mid0 + i * 7); w.append(rng.uniform(0.01, 1))
+    for i in range(f1):
+        for s in range(ns):
+            if rng.random() < 0.7 or s == 0:
+                u.append(mid0 + i); v.append(mid0 + f1 + s); w.append(rng.uniform(0.01, 1))
+    w[5] = w[6] = w[7]
+    return (n, np.array(u, np.int32), np.array(v, np.int32), np.array(w, np.float32))
+
+
+def test_wide_fanout_deviation_sort():
+    n, u, v, w = wide_fanout_graph()
+    say("case wide fanout 3000/40")
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, level, L = og.sfxt()
+    g = engine_for(n, u, v, w)
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    for k in (1, 10, 1000, 5000):
+        want, _ = og.topk(dist, succ, k)
+        g.reset()
+        g.report_paths(k)
+        assert_bits_equal(want, g.get_slacks(k), f"wide fanout K={k}")
+
+
+@pytest.mark.parametrize("blocks", [1, 16, 256])
+def test_c1_level_loop_grid_does_not_change_results(blocks):
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1)
+    say(f"case C1 level-loop grid {blocks}")
+    g = engine_for(n, u, v, w)
+    g.report_paths(10000, sfxt_blocks=blocks)
+    assert g.timings()["sfxt_blocks"] <= blocks
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    want, _ = og.topk(dist, succ, 10000)
+    assert_bits_equal(want, g.get_slacks(10000), f"C1 grid {blocks}")
