@@ -82,7 +82,7 @@ def main():
     kt = [run(kernel_timing=True) for _ in range(3)]
     sfxt_kernel_ms = statistics.mean(x["sfxt_kernel_ms"] for x in kt)
     walk_ms = statistics.mean(x["walk_count_ms"] + x["walk_fill_ms"] for x in kt)
-    walk_launches = 2 * kt[0]["walk_launches"]
+    walk_launches = kt[0]["walk_launches"]
     # deviation-walk statistics for the algorithmic byte count
     g.reset()
     g.report_paths(k, collect_stats=True)

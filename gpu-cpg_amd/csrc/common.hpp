@@ -71,4 +71,37 @@ __device__ inline uint64_t wave_reserve(unsigned long long* counter, bool want) 
   return base + __popcll(mask & lanemask_lt());
 }
 
+// Block-wide reservation: each thread asks for `want` consecutive slots on
+// a 64-bit counter with one atomic per block; returns the thread's first
+// slot.  Every thread of the block must call it (it contains barriers).
+template <int kThreads>
+__device__ inline uint64_t block_reserve(unsigned long long* counter, unsigned want) {
+  static_assert(kThreads % 64 == 0, "block must be whole waves");
+  constexpr int kW = kThreads / 64;
+  __shared__ unsigned s_wave[kW];
+  __shared__ unsigned long long s_base;
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  unsigned incl = want;
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kW; ++w) {
+      const unsigned c = s_wave[w];
+      s_wave[w] = static_cast<unsigned>(tot);
+      tot += c;
+    }
+    s_base = tot ? atomicAdd(counter, tot) : 0ull;
+  }
+  __syncthreads();
+  const uint64_t r = s_base + s_wave[wave] + (incl - want);
+  __syncthreads();
+  return r;
+}
+
 }  // namespace cpg

@@ -351,11 +351,17 @@ struct cpg_handle {
     const uint64_t batch_target = o.batch > 0 ? static_cast<uint64_t>(o.batch) : 131072ull;
     uint64_t cap = o.max_nodes > 0 ? static_cast<uint64_t>(o.max_nodes)
                                    : std::max<uint64_t>(uint64_t(1) << 22,
-                                                        8ull * static_cast<uint64_t>(k) +
+                                                        16ull * static_cast<uint64_t>(k) +
                                                             static_cast<uint64_t>(nroots) + (1u << 20));
     cap = std::min<uint64_t>(cap, 0x7fffffffull);
     int cur = 0;
     PfxtCtl c{};
+    const bool trace = std::getenv("CPG_TRACE") != nullptr;
+    // Rounds with a finite bound walk once: the fill pass stores children at
+    // the previous bound and histograms them, then the bound is tightened
+    // and compaction drops what lies above it.  A node-store overflow
+    // restarts the query with a larger store and a count pass every round.
+    bool two_pass = false;
     for (int attempt = 0;; ++attempt) {
       ensure_pfxt_buffers(cap);
       PfxtCtl init{};
@@ -378,6 +384,7 @@ struct cpg_handle {
       wa.nodes = ns; wa.hist = hist_all.p; wa.ctl = pctl.p;
       wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = g.max_abs_wgt;
       wa.collect_stats = o.collect_stats;
+      uint32_t ukey_host = kKeyInf;
       wa.step_limit = static_cast<int>(std::min<int64_t>(
           0x3fffffff, (static_cast<int64_t>(diameter) + 2) * (g.max_out_degree + 2) + 16));
       bool overflow = false;
@@ -396,20 +403,41 @@ struct cpg_handle {
           ev_fill.create();
           CPG_HIP(hipEventRecord(ev_count.a, stream));
         }
-        launch_walk(wa, false, grid, stream);
-        if (kernel_timing) CPG_HIP(hipEventRecord(ev_count.b, stream));
-        launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
-        if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.a, stream));
-        launch_walk(wa, true, grid, stream);
-        if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
+        const bool count_pass = two_pass || ukey_host == kKeyInf;
+        if (count_pass) {
+          wa.hist_fill = 0;
+          launch_walk(wa, false, grid, stream);
+          if (kernel_timing) CPG_HIP(hipEventRecord(ev_count.b, stream));
+          launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+          if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.a, stream));
+          launch_walk(wa, true, grid, stream);
+          if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
+        } else {
+          wa.hist_fill = 1;
+          if (kernel_timing) {
+            CPG_HIP(hipEventRecord(ev_count.b, stream));
+            CPG_HIP(hipEventRecord(ev_fill.a, stream));
+          }
+          launch_walk(wa, true, grid, stream);
+          if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
+          launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+        }
         launch_compact_live(l, grid, stream);
         d2h(&c, pctl.p, 1, stream);
         CPG_HIP(hipStreamSynchronize(stream));
         if (kernel_timing) {
-          timing.walk_count_ms += ev_count.ms();
-          timing.walk_fill_ms += ev_fill.ms();
-          timing.walk_launches += 1;
+          const double cm = ev_count.ms(), fm = ev_fill.ms();
+          timing.walk_count_ms += cm;
+          timing.walk_fill_ms += fm;
+          timing.walk_launches += count_pass ? 2 : 1;
+          if (trace)
+            std::fprintf(stderr,
+                         "round %d %s batch %llu expanded %llu nodes %llu live %llu U %.9g "
+                         "count %.3f ms fill %.3f ms\n",
+                         r, count_pass ? "count+fill" : "fill", c.batch_n, c.expanded, c.node_n,
+                         c.live_next_n, static_cast<double>(key_float(c.ukey)), cm, fm);
         }
+        ukey_host = c.ukey;
         if (c.err) throw std::runtime_error("PFXT walk exceeded its step bound");
         if (c.overflow) { overflow = true; break; }
         if (c.done) break;  // the round did nothing: live stays in `cur`
@@ -418,6 +446,7 @@ struct cpg_handle {
       if (!overflow) break;
       if (attempt >= 4 || cap >= 0x7fffffffull) throw std::runtime_error("prefix-tree node capacity exceeded");
       cap = std::min<uint64_t>(cap * 2, 0x7fffffffull);
+      two_pass = true;
       timing.retries++;
     }
     timing.rounds = c.rounds;

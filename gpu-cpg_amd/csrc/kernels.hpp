@@ -102,6 +102,7 @@ struct WalkArgs {
   float max_abs_distf;
   float max_abs_wgt;
   int collect_stats;
+  int hist_fill;          // fill pass also histograms the children (no count pass this round)
   int step_limit;         // bound on one walk's steps: (levels + 2) * (max out-degree + 2)
 };
 

@@ -127,6 +127,8 @@ __global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, u
   if (threadIdx.x == 0) {
     ctl->batch_n = 0;
     ctl->child_n = 0;
+    ctl->fill_node_base = ctl->node_n;
+    ctl->fill_live_base = ctl->live_n;
   }
   unsigned long long total = 0;
   const uint32_t b = find_crossing(hist_pend, target, &total);
@@ -144,32 +146,65 @@ __global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, u
   clear_hist(hist_pend);
 }
 
+constexpr int kTileItems = 8;  // items per thread per tile in the filter kernels
+
+// Marks the live paths with key <= thkey that are not expanded yet and
+// appends them to the batch; one counter atomic per block and tile.
 __global__ void __launch_bounds__(kBlock) make_batch_kernel(LiveArgs a) {
   PfxtCtl* ctl = a.ctl;
   if (ctl->done) return;
   const uint64_t n = ctl->live_n;
   const uint32_t th = ctl->thkey;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
-       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t i = base + threadIdx.x;
-    bool take = false;
-    uint32_t id = 0;
-    if (i < n) {
-      id = a.idx[i];
-      take = !(id & kExpandedBit) && a.key[i] <= th;
+  constexpr uint64_t kTile = static_cast<uint64_t>(kBlock) * kTileItems;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile; base < n;
+       base += static_cast<uint64_t>(gridDim.x) * kTile) {
+    uint32_t id[kTileItems];
+    unsigned mask = 0, cnt = 0;
+#pragma unroll
+    for (int q = 0; q < kTileItems; ++q) {
+      const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
+      id[q] = 0;
+      if (i < n) {
+        id[q] = a.idx[i];
+        if (!(id[q] & kExpandedBit) && a.key[i] <= th) {
+          mask |= 1u << q;
+          ++cnt;
+        }
+      }
     }
-    const uint64_t slot = wave_reserve(&ctl->batch_n, take);
-    if (take) {
-      a.idx[i] = id | kExpandedBit;
-      a.batch[slot] = id;
+    uint64_t slot = block_reserve<kBlock>(&ctl->batch_n, cnt);
+#pragma unroll
+    for (int q = 0; q < kTileItems; ++q) {
+      if (mask & (1u << q)) {
+        const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
+        a.idx[i] = id[q] | kExpandedBit;
+        a.batch[slot++] = id[q];
+      }
     }
   }
 }
 
+constexpr int kStage = 256;  // per-wave LDS slots for children awaiting a flush
+
+// Walks the suffix chains of the batch's parents and finds every child
+// with cost <= U.  Count pass (kFill = false): histogram the children's
+// keys.  Fill pass: store them (node store + live buffer, after the live
+// entries present when the round began); with a.hist_fill it also
+// histograms them, for rounds that skip the count pass.  Children are
+// staged per wave in LDS and written with one counter atomic per flush.
 template <bool kFill>
 __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
+  constexpr int kWaves = kBlock / 64;
+  constexpr int kS = kFill ? kStage : 1;
+  __shared__ float s_cost[kWaves][kS];
+  __shared__ int s_to[kWaves][kS];
+  __shared__ int s_from[kWaves][kS];
+  __shared__ int s_parent[kWaves][kS];
+  __shared__ int s_level[kWaves][kS];
   PfxtCtl* ctl = a.ctl;
   if (ctl->done) return;
+  const int wave = threadIdx.x >> 6;
+  const int lane = __lane_id();
   const uint64_t nb = ctl->batch_n;
   const uint32_t ukey = ctl->ukey;
   const bool bounded = ukey != kKeyInf;
@@ -180,6 +215,38 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
   const uint64_t live_base = kFill ? ctl->fill_live_base : 0;
   const float mag = 2.0f * a.max_abs_distf + a.max_abs_wgt + 1.0f;
   unsigned long long hops = 0, devv = 0, counted = 0;
+  int staged = 0;  // wave-uniform
+
+  // Writes the wave's staged children: one atomic on node_n, then each
+  // lane stores every 64th child.  All lanes of the wave call it together.
+  auto flush = [&]() {
+    if (!kFill || staged == 0) return;
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&ctl->node_n, static_cast<unsigned long long>(staged));
+    base = __shfl(base, 0);
+    for (int q = lane; q < staged; q += 64) {
+      const uint64_t slot = base + q;
+      const float cost = s_cost[wave][q];
+      if (slot < cap) {
+        a.nodes.cost[slot] = cost;
+        a.nodes.to[slot] = s_to[wave][q];
+        a.nodes.from[slot] = s_from[wave][q];
+        a.nodes.parent[slot] = s_parent[wave][q];
+        a.nodes.level[slot] = s_level[wave][q];
+        const uint64_t ls = live_base + (slot - node_base);
+        const uint32_t k = float_key(cost);
+        a.live_key[ls] = k;
+        a.live_idx[ls] = static_cast<uint32_t>(slot);
+        if (a.hist_fill && !key_is_zero(k)) atomicAdd(&a.hist[key_bin(k, klo, shift)], 1u);
+      } else {
+        atomicExch(&ctl->overflow, 1);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    staged = 0;
+  };
+
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < nb;
        base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t i = base + threadIdx.x;
@@ -207,6 +274,7 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
         atomicExch(&ctl->err, 1);
         mode = 2;
       }
+      int from = v;
       if (mode == 0) {
         const int4 r = a.vrec[v];
         ++hops;
@@ -243,20 +311,19 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
         }
       }
       if (kFill) {
-        const uint64_t slot = wave_reserve(&ctl->node_n, emit);
-        if (emit) {
-          if (slot < cap) {
-            a.nodes.cost[slot] = cc;
-            a.nodes.to[slot] = cw;
-            a.nodes.from[slot] = v;
-            a.nodes.parent[slot] = p;
-            a.nodes.level[slot] = lvl + 1;
-            const uint64_t ls = live_base + (slot - node_base);
-            a.live_key[ls] = float_key(cc);
-            a.live_idx[ls] = static_cast<uint32_t>(slot);
-          } else {
-            atomicExch(&ctl->overflow, 1);
+        const uint64_t m = __ballot(emit);
+        if (m) {
+          const int add = static_cast<int>(__popcll(m));
+          if (staged + add > kS) flush();
+          if (emit) {
+            const int q = staged + static_cast<int>(__popcll(m & lanemask_lt()));
+            s_cost[wave][q] = cc;
+            s_to[wave][q] = cw;
+            s_from[wave][q] = from;
+            s_parent[wave][q] = p;
+            s_level[wave][q] = lvl + 1;
           }
+          staged += add;
         }
       } else if (emit) {
         ++counted;
@@ -265,16 +332,17 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
       }
     }
   }
+  flush();
   if (!kFill) {
     for (int o = 32; o > 0; o >>= 1) counted += __shfl_xor(counted, o);
-    if (__lane_id() == 0 && counted) atomicAdd(&ctl->child_n, counted);
+    if (lane == 0 && counted) atomicAdd(&ctl->child_n, counted);
   }
   if (a.collect_stats) {
     for (int o = 32; o > 0; o >>= 1) {
       hops += __shfl_xor(hops, o);
       devv += __shfl_xor(devv, o);
     }
-    if (__lane_id() == 0) {
+    if (lane == 0) {
       atomicAdd(&ctl->hops, hops);
       atomicAdd(&ctl->devvis, devv);
     }
@@ -294,8 +362,6 @@ __global__ void __launch_bounds__(kSelBlock) select_bound_kernel(PfxtCtl* ctl, u
     }
     ctl->ukey = uk;
     ctl->shift = window_shift(ctl->klo, uk);
-    ctl->fill_node_base = ctl->node_n;
-    ctl->fill_live_base = ctl->live_n;
     ctl->swap_pending = 1;
     ctl->expanded += ctl->batch_n;
     ctl->rounds += 1;
@@ -304,6 +370,9 @@ __global__ void __launch_bounds__(kSelBlock) select_bound_kernel(PfxtCtl* ctl, u
   clear_hist(hist_all);
 }
 
+// Keeps the live paths (old entries and this round's children) with key
+// <= U and rebuilds both histograms for the next round.  Survivors keep
+// their order within a block's tile; tiles land in reservation order.
 __global__ void __launch_bounds__(kBlock) compact_live_kernel(LiveArgs a) {
   PfxtCtl* ctl = a.ctl;
   if (ctl->done) return;
@@ -312,23 +381,36 @@ __global__ void __launch_bounds__(kBlock) compact_live_kernel(LiveArgs a) {
   const uint64_t n = ctl->fill_live_base + (nn - ctl->fill_node_base);
   const uint32_t uk = ctl->ukey;
   const uint32_t klo = ctl->klo, shift = ctl->shift;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
-       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t i = base + threadIdx.x;
-    bool keep = false;
-    uint32_t k = 0, id = 0;
-    if (i < n) {
-      k = a.key[i];
-      id = a.idx[i];
-      keep = k <= uk;
+  constexpr uint64_t kTile = static_cast<uint64_t>(kBlock) * kTileItems;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile; base < n;
+       base += static_cast<uint64_t>(gridDim.x) * kTile) {
+    uint32_t k[kTileItems], id[kTileItems];
+    unsigned mask = 0, cnt = 0;
+#pragma unroll
+    for (int q = 0; q < kTileItems; ++q) {
+      const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
+      k[q] = 0;
+      id[q] = 0;
+      if (i < n) {
+        k[q] = a.key[i];
+        id[q] = a.idx[i];
+        if (k[q] <= uk) {
+          mask |= 1u << q;
+          ++cnt;
+        }
+      }
     }
-    const uint64_t slot = wave_reserve(&ctl->live_next_n, keep);
-    if (keep) {
-      a.out_key[slot] = k;
-      a.out_idx[slot] = id;
-      const uint32_t b = key_bin(k, klo, shift);
-      if (!key_is_zero(k)) atomicAdd(&a.hist_all[b], 1u);
-      if (!(id & kExpandedBit)) atomicAdd(&a.hist_pend[b], 1u);
+    uint64_t slot = block_reserve<kBlock>(&ctl->live_next_n, cnt);
+#pragma unroll
+    for (int q = 0; q < kTileItems; ++q) {
+      if (mask & (1u << q)) {
+        a.out_key[slot] = k[q];
+        a.out_idx[slot] = id[q];
+        ++slot;
+        const uint32_t b = key_bin(k[q], klo, shift);
+        if (!key_is_zero(k[q])) atomicAdd(&a.hist_all[b], 1u);
+        if (!(id[q] & kExpandedBit)) atomicAdd(&a.hist_pend[b], 1u);
+      }
     }
   }
 }

@@ -110,6 +110,8 @@ inline T __shfl_xor(T v, int m, int = 64) {
   return simt_from_bits<T>(s[(simt::lane_id() ^ m) & 63]);
 }
 inline void __syncthreads() { simt::block_sync(); }
+// every live lane of the wave meets here, as the wave does in lockstep
+inline void __builtin_amdgcn_wave_barrier() { (void)simt::wave_sync(0); }
 
 template <class T>
 inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }

@@ -169,3 +169,17 @@ def test_c1_level_loop_grid_does_not_change_results(blocks):
     np.testing.assert_array_equal(gl, level)
     want, _ = og.topk(dist, succ, 10000)
     assert_bits_equal(want, g.get_slacks(10000), f"C1 grid {blocks}")
+
+
+@pytest.mark.parametrize("k,max_nodes", [(10000, 12000), (50000, 60000)])
+def test_c1_node_store_overflow_restarts_exactly(k, max_nodes):
+    """A node store too small for the single-walk rounds overflows; the
+    query restarts with a larger store and a count pass every round, and
+    the result is still exact."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1)
+    say(f"case C1 overflow K={k} max_nodes={max_nodes}")
+    g = engine_for(n, u, v, w)
+    g.report_paths(k, max_nodes=max_nodes)
+    assert g.timings()["retries"] >= 1
+    want, _ = og.topk(dist, succ, k)
+    assert_bits_equal(want, g.get_slacks(k), f"C1 overflow K={k}")
