@@ -26,7 +26,8 @@ class QueryOpts(ctypes.Structure):
                 ("root_tie_quirk", ctypes.c_int), ("batch", ctypes.c_int64),
                 ("max_nodes", ctypes.c_int64), ("mem_budget_bytes", ctypes.c_int64),
                 ("endpoint_mask", ctypes.c_void_p), ("collect_stats", ctypes.c_int),
-                ("kernel_timing", ctypes.c_int), ("sfxt_blocks", ctypes.c_int)]
+                ("kernel_timing", ctypes.c_int), ("sfxt_blocks", ctypes.c_int),
+                ("bound_probe", ctypes.c_int)]
 
 
 class Timing(ctypes.Structure):
@@ -38,7 +39,8 @@ class Timing(ctypes.Structure):
                 ("roots_dropped", ctypes.c_int64), ("retries", ctypes.c_int64),
                 ("sfxt_kernel_ms", ctypes.c_double), ("walk_count_ms", ctypes.c_double),
                 ("walk_fill_ms", ctypes.c_double), ("walk_launches", ctypes.c_int64),
-                ("sfxt_levels_ms", ctypes.c_double), ("sfxt_blocks", ctypes.c_int64)]
+                ("sfxt_levels_ms", ctypes.c_double), ("sfxt_blocks", ctypes.c_int64),
+                ("probes", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -221,7 +223,7 @@ class CpGen:
 
     def report_paths(self, k, *args, root_tie_quirk=True, batch=0, max_nodes=0,
                      collect_stats=False, init_split_perc=0.005, kernel_timing=False,
-                     sfxt_blocks=0, **kwargs):
+                     sfxt_blocks=0, bound_probe=0, **kwargs):
         """Runs the GPG query for the k lowest-cost paths.  Positional
         arguments after k (max_dev_lvls, enable_compress, methods, ...) are
         accepted for call compatibility with the reference; the engine always
@@ -235,6 +237,7 @@ class CpGen:
         o.init_split_perc = float(init_split_perc)
         o.kernel_timing = 1 if kernel_timing else 0
         o.sfxt_blocks = int(sfxt_blocks)
+        o.bound_probe = int(bound_probe)
         _c(self, self.L.cpg_query(self.h, int(k), ctypes.byref(o)))
         self._k = int(k)
 

@@ -289,6 +289,75 @@ struct cpg_handle {
     node_cap = cap;
   }
 
+  // Finds a finite first bound without walking the roots' whole chains:
+  // count the roots and their children up to a trial bound c_min + g with a
+  // pruned count pass, and widen g until at least k nonzero paths fall
+  // under it; the bound is then the upper key of the histogram bin where
+  // the running count reaches k (k distinct paths cost no more, so no path
+  // above it can be among the k cheapest).  Returns the bound's key, or
+  // kKeyInf when eight trials fall short or the roots and their children
+  // are fewer than k (the rounds then start unbounded).
+  uint32_t probe_bound(WalkArgs& wa, int64_t k, const std::vector<uint8_t>& kp,
+                       const std::vector<int>& sd, float cmin, uint32_t klo, bool trace) {
+    std::vector<float> rc;
+    for (size_t i = 0; i < sd.size(); ++i)
+      if (kp[i]) rc.push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+    std::sort(rc.begin(), rc.end());
+    const uint64_t nb = rc.size();
+    launch_iota(batch.p, nb, grid, stream);
+    const float p10 = rc[rc.size() / 10];
+    double gap = static_cast<double>(p10) - static_cast<double>(cmin);
+    gap = std::max({gap, std::fabs(static_cast<double>(cmin)) * 0x1p-8,
+                    static_cast<double>(max_abs_distf) * 0x1p-8, 1e-6});
+    wa.hist_fill = 0;
+    uint64_t prev_total = 0;
+    int t = 0;
+    for (; t < 8; ++t) {
+      const float u0 = static_cast<float>(static_cast<double>(cmin) + gap);
+      const uint32_t uk = float_key(u0);
+      launch_probe_setup(pctl.p, uk, window_shift(klo, uk), nb, stream);
+      CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * kHistBins, stream));
+      launch_walk(wa, false, grid, stream);
+      PfxtCtl pc{};
+      d2h(&pc, pctl.p, 1, stream);
+      CPG_HIP(hipStreamSynchronize(stream));
+      if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
+      uint64_t roots_le = 0;
+      for (float c : rc)
+        if (c <= u0 && c != 0.0f) ++roots_le;
+      const uint64_t total = roots_le + pc.child_n;
+      if (trace)
+        std::fprintf(stderr, "probe %d bound %.9g roots %llu children %llu\n", t,
+                     static_cast<double>(u0), static_cast<unsigned long long>(roots_le),
+                     static_cast<unsigned long long>(pc.child_n));
+      if (total >= static_cast<uint64_t>(k)) {
+        // the roots join the children in the histogram, then the crossing
+        LiveArgs la{live_key[0].p, live_idx[0].p, nullptr, nullptr, batch.p, hist_all.p,
+                    hist_pend.p, pctl.p};
+        launch_hist_live(la, grid, stream);
+        CPG_HIP(hipMemsetAsync(hist_pend.p, 0, sizeof(unsigned) * kHistBins, stream));
+        launch_probe_finish(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+        d2h(&pc, pctl.p, 1, stream);
+        CPG_HIP(hipStreamSynchronize(stream));
+        timing.probes = t + 1;
+        return pc.ukey;
+      }
+      // a wider bound that counts nothing new means every child of every
+      // root is already counted: k needs deeper paths, stop probing
+      if (t > 0 && total == prev_total) break;
+      prev_total = total;
+      // counts grow steeply with the bound (children cluster in cost), so
+      // widen by at most 2x: the last, successful trial then costs at most
+      // a pruned pass at twice the needed width
+      const double f = std::sqrt(static_cast<double>(k) / static_cast<double>(std::max<uint64_t>(total, 1))) * 1.1;
+      gap *= std::min(2.0, std::max(1.25, f));
+    }
+    timing.probes = std::min(t + 1, 8);
+    launch_probe_setup(pctl.p, kKeyInf, window_shift(klo, kKeyInf), 0, stream);
+    CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * kHistBins, stream));
+    return kKeyInf;
+  }
+
   void query(int64_t k, const cpg_query_opts& o) {
     if (!loaded) throw std::runtime_error("no graph loaded");
     have_result = false;
@@ -376,17 +445,21 @@ struct cpg_handle {
       launch_roots(srcs.p, keep.p, nsrc, dist.p, distf.p, ns, live_key[0].p, live_idx[0].p, pctl.p,
                    stream);
       cur = 0;
-      LiveArgs la{live_key[0].p, live_idx[0].p, nullptr, nullptr, batch.p, hist_all.p, hist_pend.p,
-                  pctl.p};
-      launch_hist_live(la, grid, stream);
       WalkArgs wa{};
       wa.vrec = vrec.p; wa.devinfo = devinfo.p; wa.devs = devs.p; wa.batch = batch.p;
       wa.nodes = ns; wa.hist = hist_all.p; wa.ctl = pctl.p;
       wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = g.max_abs_wgt;
       wa.collect_stats = o.collect_stats;
-      uint32_t ukey_host = kKeyInf;
       wa.step_limit = static_cast<int>(std::min<int64_t>(
           0x3fffffff, (static_cast<int64_t>(diameter) + 2) * (g.max_out_degree + 2) + 16));
+      const int64_t nkept = nroots - dropped;
+      const bool probe = nkept > 0 && (o.bound_probe > 0 ||
+                                       (o.bound_probe == 0 && nkept * (diameter + 1) > (1 << 20)));
+      uint32_t ukey_host = kKeyInf;
+      if (probe) ukey_host = probe_bound(wa, k, kp, sd, minc, init.klo, trace);
+      LiveArgs la{live_key[0].p, live_idx[0].p, nullptr, nullptr, batch.p, hist_all.p, hist_pend.p,
+                  pctl.p};
+      launch_hist_live(la, grid, stream);
       bool overflow = false;
       for (int r = 0;; ++r) {
         if (r > 100000) throw std::runtime_error("PFXT round limit exceeded");

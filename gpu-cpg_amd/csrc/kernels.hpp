@@ -121,6 +121,10 @@ void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dis
                   const float* distf, NodeStore nodes, uint32_t* live_key, uint32_t* live_idx,
                   PfxtCtl* ctl, hipStream_t s);
 void launch_round_begin(PfxtCtl* ctl, hipStream_t s);
+void launch_probe_setup(PfxtCtl* ctl, uint32_t ukey, uint32_t shift, unsigned long long nb,
+                        hipStream_t s);
+void launch_iota(uint32_t* p, uint64_t n, int grid, hipStream_t s);
+void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s);
 void launch_hist_live(const LiveArgs& a, int grid, hipStream_t s);
 void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long batch_target,
                          hipStream_t s);

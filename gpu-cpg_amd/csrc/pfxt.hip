@@ -415,6 +415,45 @@ __global__ void __launch_bounds__(kBlock) compact_live_kernel(LiveArgs a) {
   }
 }
 
+// Bound probe: the walk's inputs for one trial bound ukey over the roots
+// (node slots 0..nb-1, listed in batch by iota_kernel).
+__global__ void probe_setup_kernel(PfxtCtl* ctl, uint32_t ukey, uint32_t shift,
+                                   unsigned long long nb) {
+  ctl->ukey = ukey;
+  ctl->shift = shift;
+  ctl->batch_n = nb;
+  ctl->child_n = 0;
+  ctl->done = 0;
+}
+
+__global__ void iota_kernel(uint32_t* p, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = static_cast<uint32_t>(i);
+}
+
+// Bound probe succeeded (at least k nonzero paths with cost <= the trial
+// bound were counted): the bound becomes the upper key of the bin where
+// the running count reaches k, and the probe's batch is dropped.
+__global__ void __launch_bounds__(kSelBlock) probe_finish_kernel(PfxtCtl* ctl, unsigned* hist_all,
+                                                                 unsigned long long k) {
+  unsigned long long total = 0;
+  const uint32_t b = find_crossing(hist_all, k, &total);
+  if (threadIdx.x == 0) {
+    uint32_t uk = ctl->ukey;
+    if (b < static_cast<uint32_t>(kHistBins)) {
+      const uint32_t up = bin_upper_key(b, ctl->klo, ctl->shift);
+      if (up < uk) uk = up;
+    }
+    ctl->ukey = uk;
+    ctl->shift = window_shift(ctl->klo, uk);
+    ctl->batch_n = 0;
+    ctl->child_n = 0;
+  }
+  __syncthreads();
+  clear_hist(hist_all);
+}
+
 __global__ void round_begin_kernel(PfxtCtl* ctl) {
   if (ctl->done) return;
   if (ctl->swap_pending) {
@@ -433,6 +472,23 @@ void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dis
   const int grid = (nsrc + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(roots_kernel, dim3(grid), dim3(kBlock), 0, s, srcs, keep, nsrc, dist, distf, nodes, live_key, live_idx,
                                        ctl);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_probe_setup(PfxtCtl* ctl, uint32_t ukey, uint32_t shift, unsigned long long nb,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(probe_setup_kernel, dim3(1), dim3(1), 0, s, ctl, ukey, shift, nb);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_iota(uint32_t* p, uint64_t n, int grid, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(iota_kernel, dim3(grid), dim3(kBlock), 0, s, p, n);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s) {
+  hipLaunchKernelGGL(probe_finish_kernel, dim3(1), dim3(kSelBlock), 0, s, ctl, hist_all, k);
   CPG_HIP(hipGetLastError());
 }
 

@@ -53,6 +53,9 @@ typedef struct {
   int kernel_timing;        /* 1: time the SFXT and walk kernels with HIP events
                                on the engine's stream (cpg_timing.*_kernel_ms) */
   int sfxt_blocks;          /* SFXT level-loop grid in blocks (0 = automatic) */
+  int bound_probe;          /* first bound from pruned count passes: 0 = when the
+                               roots' chains are long (automatic), 1 = always,
+                               -1 = never (the first round walks unbounded)   */
 } cpg_query_opts;
 
 /* Timings of the last query, milliseconds (gpucpg/gpucpg.cuh:327-356). */
@@ -75,6 +78,7 @@ typedef struct {
   int64_t walk_launches;  /* kernel_timing: walk kernel launches, both passes            */
   double sfxt_levels_ms;  /* kernel_timing: SFXT init + level loop (part of sfxt_kernel_ms)  */
   int64_t sfxt_blocks;    /* SFXT level-loop grid used                                    */
+  int64_t probes;         /* bound-probe count passes run before the first round           */
 } cpg_timing;
 
 /* ---- lifetime ---- */

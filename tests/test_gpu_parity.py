@@ -183,3 +183,38 @@ def test_c1_node_store_overflow_restarts_exactly(k, max_nodes):
     assert g.timings()["retries"] >= 1
     want, _ = og.topk(dist, succ, k)
     assert_bits_equal(want, g.get_slacks(k), f"C1 overflow K={k}")
+
+
+@pytest.mark.parametrize("probe", [1, -1])
+@pytest.mark.parametrize("name", ["simple.txt", "dac21_paper_example.txt", "tv80_unit_wgts.txt"])
+def test_reference_benchmarks_bound_probe_modes(golden_dir, name, probe):
+    """The first bound from pruned count passes (probe=1) or from an
+    unbounded first round (probe=-1) gives the same paths."""
+    path = os.path.join(golden_dir, name)
+    say(f"case benchmark {name} bound_probe={probe}")
+    og = pyoracle.OracleGraph.from_text(path)
+    dist, succ, level, L = og.sfxt()
+    g = cpgamd.CpGen()
+    g.read_input(path)
+    for k in (1, 5, 1000):
+        want, _ = og.topk(dist, succ, k)
+        g.reset()
+        g.report_paths(k, bound_probe=probe)
+        got = g.get_slacks(k)
+        if name.startswith("tv80"):
+            np.testing.assert_array_equal(np.sort(got), np.sort(want))
+        else:
+            assert_bits_equal(want, got, f"{name} K={k} probe={probe}")
+
+
+@pytest.mark.parametrize("probe", [1, -1])
+@pytest.mark.parametrize("k", [100, 10000])
+def test_c1_bound_probe_modes(k, probe):
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1)
+    say(f"case C1 K={k} bound_probe={probe}")
+    g = engine_for(n, u, v, w)
+    g.report_paths(k, bound_probe=probe)
+    if probe < 0:
+        assert g.timings()["probes"] == 0
+    want, _ = og.topk(dist, succ, k)
+    assert_bits_equal(want, g.get_slacks(k), f"C1 K={k} probe={probe}")
