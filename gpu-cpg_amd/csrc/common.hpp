@@ -15,7 +15,7 @@ constexpr uint32_t kExpandedBit = 0x80000000u;  // live-entry flag: parent alrea
 constexpr uint32_t kKeyInf = 0xFF800000u;       // orderable key of +inf ("no bound")
 constexpr uint32_t kKeyPosZero = 0x80000000u;   // key of +0.0f
 constexpr uint32_t kKeyNegZero = 0x7FFFFFFFu;   // key of -0.0f
-constexpr int kHistBins = 65536;                // threshold histograms
+constexpr int kHistBins = 4096;                 // threshold histograms
 
 #define CPG_HIP(call)                                                          \
   do {                                                                         \

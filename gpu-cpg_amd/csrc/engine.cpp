@@ -115,8 +115,9 @@ struct cpg_handle {
   // PFXT
   DevBuf<float> n_cost;
   DevBuf<int> n_to, n_from, n_parent, n_level;
-  DevBuf<uint32_t> live_key[2], live_idx[2], batch;
-  DevBuf<unsigned> hist_all, hist_pend;
+  DevBuf<uint32_t> n_key, batch;
+  DevBuf<uint8_t> n_flag;
+  DevBuf<unsigned> hist_all, hist_pend, hist_part;
   DevBuf<PfxtCtl> pctl;
   DevBuf<uint8_t> keep;
   uint64_t node_cap = 0;
@@ -125,7 +126,7 @@ struct cpg_handle {
   DevBuf<uint64_t> skey[2];
   DevBuf<uint32_t> sval[2];
   DevBuf<unsigned> ghist;
-  DevBuf<unsigned long long> zc;
+  DevBuf<unsigned long long> nsel;
   DevBuf<float> r_cost;
   DevBuf<int> r_level, r_from, r_to, r_pidx, r_parent, rank_of;
   int64_t nres = 0;
@@ -281,10 +282,11 @@ struct cpg_handle {
     if (cap > 0x7fffffffull) throw std::runtime_error("node capacity exceeds 2^31");
     n_cost.reserve(cap); n_to.reserve(cap); n_from.reserve(cap);
     n_parent.reserve(cap); n_level.reserve(cap);
-    for (int i = 0; i < 2; ++i) { live_key[i].reserve(cap); live_idx[i].reserve(cap); }
+    n_key.reserve(cap); n_flag.reserve(cap);
     batch.reserve(cap);
     hist_all.reserve(kHistBins);
     hist_pend.reserve(kHistBins);
+    hist_part.reserve(2 * static_cast<size_t>(kHistParts) * kHistBins);
     pctl.reserve(1);
     node_cap = cap;
   }
@@ -297,6 +299,17 @@ struct cpg_handle {
   // above it can be among the k cheapest).  Returns the bound's key, or
   // kKeyInf when eight trials fall short or the roots and their children
   // are fewer than k (the rounds then start unbounded).
+  NodeStore node_store() {
+    return NodeStore{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, n_key.p, n_flag.p, node_cap};
+  }
+
+  // Histograms of the stored paths at or below the bound (see launch_hist_nodes).
+  void hist_nodes(int mode, bool with_pend, bool accumulate) {
+    launch_hist_nodes(node_store(), pctl.p, mode, with_pend, hist_part.p,
+                      hist_part.p + static_cast<size_t>(kHistParts) * kHistBins, hist_all.p,
+                      hist_pend.p, accumulate, stream);
+  }
+
   uint32_t probe_bound(WalkArgs& wa, int64_t k, const std::vector<uint8_t>& kp,
                        const std::vector<int>& sd, float cmin, uint32_t klo, bool trace) {
     std::vector<float> rc;
@@ -309,7 +322,6 @@ struct cpg_handle {
     double gap = static_cast<double>(p10) - static_cast<double>(cmin);
     gap = std::max({gap, std::fabs(static_cast<double>(cmin)) * 0x1p-8,
                     static_cast<double>(max_abs_distf) * 0x1p-8, 1e-6});
-    wa.hist_fill = 0;
     uint64_t prev_total = 0;
     int t = 0;
     for (; t < 8; ++t) {
@@ -332,10 +344,7 @@ struct cpg_handle {
                      static_cast<unsigned long long>(pc.child_n));
       if (total >= static_cast<uint64_t>(k)) {
         // the roots join the children in the histogram, then the crossing
-        LiveArgs la{live_key[0].p, live_idx[0].p, nullptr, nullptr, batch.p, hist_all.p,
-                    hist_pend.p, pctl.p};
-        launch_hist_live(la, grid, stream);
-        CPG_HIP(hipMemsetAsync(hist_pend.p, 0, sizeof(unsigned) * kHistBins, stream));
+        hist_nodes(0, false, true);
         launch_probe_finish(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
         d2h(&pc, pctl.p, 1, stream);
         CPG_HIP(hipStreamSynchronize(stream));
@@ -423,13 +432,13 @@ struct cpg_handle {
                                                         16ull * static_cast<uint64_t>(k) +
                                                             static_cast<uint64_t>(nroots) + (1u << 20));
     cap = std::min<uint64_t>(cap, 0x7fffffffull);
-    int cur = 0;
     PfxtCtl c{};
     const bool trace = std::getenv("CPG_TRACE") != nullptr;
     // Rounds with a finite bound walk once: the fill pass stores children at
-    // the previous bound and histograms them, then the bound is tightened
-    // and compaction drops what lies above it.  A node-store overflow
-    // restarts the query with a larger store and a count pass every round.
+    // the bound, their histogram joins the stored paths', and the bound is
+    // tightened.  A node-store overflow restarts the query with a larger
+    // store and a count pass every round, which stores only children at or
+    // below the tightened bound.
     bool two_pass = false;
     for (int attempt = 0;; ++attempt) {
       ensure_pfxt_buffers(cap);
@@ -439,12 +448,8 @@ struct cpg_handle {
       init.shift = window_shift(init.klo, kKeyInf);
       init.node_cap = cap;
       h2d(pctl.p, &init, 1, stream);
-      CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * kHistBins, stream));
-      CPG_HIP(hipMemsetAsync(hist_pend.p, 0, sizeof(unsigned) * kHistBins, stream));
-      NodeStore ns{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, cap};
-      launch_roots(srcs.p, keep.p, nsrc, dist.p, distf.p, ns, live_key[0].p, live_idx[0].p, pctl.p,
-                   stream);
-      cur = 0;
+      const NodeStore ns = node_store();
+      launch_roots(srcs.p, keep.p, nsrc, dist.p, distf.p, ns, pctl.p, stream);
       WalkArgs wa{};
       wa.vrec = vrec.p; wa.devinfo = devinfo.p; wa.devs = devs.p; wa.batch = batch.p;
       wa.nodes = ns; wa.hist = hist_all.p; wa.ctl = pctl.p;
@@ -457,20 +462,13 @@ struct cpg_handle {
                                        (o.bound_probe == 0 && nkept * (diameter + 1) > (1 << 20)));
       uint32_t ukey_host = kKeyInf;
       if (probe) ukey_host = probe_bound(wa, k, kp, sd, minc, init.klo, trace);
-      LiveArgs la{live_key[0].p, live_idx[0].p, nullptr, nullptr, batch.p, hist_all.p, hist_pend.p,
-                  pctl.p};
-      launch_hist_live(la, grid, stream);
       bool overflow = false;
       for (int r = 0;; ++r) {
         if (r > 100000) throw std::runtime_error("PFXT round limit exceeded");
         if (now_ms() - tq0 > 120000.0) throw std::runtime_error("PFXT query exceeded 120 s");
-        LiveArgs l{live_key[cur].p, live_idx[cur].p, live_key[cur ^ 1].p, live_idx[cur ^ 1].p,
-                   batch.p, hist_all.p, hist_pend.p, pctl.p};
-        launch_round_begin(pctl.p, stream);
+        hist_nodes(0, true, false);
         launch_select_batch(pctl.p, hist_pend.p, r == 0 ? ~0ull : batch_target, stream);
-        launch_make_batch(l, grid, stream);
-        wa.live_key = live_key[cur].p;
-        wa.live_idx = live_idx[cur].p;
+        launch_make_batch(ns, pctl.p, batch.p, grid, stream);
         if (kernel_timing) {
           ev_count.create();
           ev_fill.create();
@@ -478,7 +476,6 @@ struct cpg_handle {
         }
         const bool count_pass = two_pass || ukey_host == kKeyInf;
         if (count_pass) {
-          wa.hist_fill = 0;
           launch_walk(wa, false, grid, stream);
           if (kernel_timing) CPG_HIP(hipEventRecord(ev_count.b, stream));
           launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
@@ -486,16 +483,15 @@ struct cpg_handle {
           launch_walk(wa, true, grid, stream);
           if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
         } else {
-          wa.hist_fill = 1;
           if (kernel_timing) {
             CPG_HIP(hipEventRecord(ev_count.b, stream));
             CPG_HIP(hipEventRecord(ev_fill.a, stream));
           }
           launch_walk(wa, true, grid, stream);
           if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
+          hist_nodes(1, false, true);
           launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
         }
-        launch_compact_live(l, grid, stream);
         d2h(&c, pctl.p, 1, stream);
         CPG_HIP(hipStreamSynchronize(stream));
         if (kernel_timing) {
@@ -505,16 +501,15 @@ struct cpg_handle {
           timing.walk_launches += count_pass ? 2 : 1;
           if (trace)
             std::fprintf(stderr,
-                         "round %d %s batch %llu expanded %llu nodes %llu live %llu U %.9g "
+                         "round %d %s batch %llu expanded %llu nodes %llu U %.9g "
                          "count %.3f ms fill %.3f ms\n",
                          r, count_pass ? "count+fill" : "fill", c.batch_n, c.expanded, c.node_n,
-                         c.live_next_n, static_cast<double>(key_float(c.ukey)), cm, fm);
+                         static_cast<double>(key_float(c.ukey)), cm, fm);
         }
         ukey_host = c.ukey;
         if (c.err) throw std::runtime_error("PFXT walk exceeded its step bound");
         if (c.overflow) { overflow = true; break; }
-        if (c.done) break;  // the round did nothing: live stays in `cur`
-        cur ^= 1;
+        if (c.done) break;  // no unexpanded path at or below the bound
       }
       if (!overflow) break;
       if (attempt >= 4 || cap >= 0x7fffffffull) throw std::runtime_error("prefix-tree node capacity exceeded");
@@ -530,31 +525,30 @@ struct cpg_handle {
     const double ts0 = now_ms();
     timing.pfxt_ms = ts0 - tp0;
 
-    // final selection over the live set (all paths with cost <= bound)
-    const uint64_t nl = c.live_n;
-    skey[0].reserve(nl); skey[1].reserve(nl); sval[0].reserve(nl); sval[1].reserve(nl);
+    // final selection: every stored path with a nonzero cost <= bound
+    const uint64_t nn = std::min<uint64_t>(c.node_n, node_cap);
+    skey[0].reserve(std::max<uint64_t>(1, nn)); skey[1].reserve(std::max<uint64_t>(1, nn));
+    sval[0].reserve(std::max<uint64_t>(1, nn)); sval[1].reserve(std::max<uint64_t>(1, nn));
+    nsel.reserve(1);
+    CPG_HIP(hipMemsetAsync(nsel.p, 0, sizeof(unsigned long long), stream));
+    const NodeStore ns = node_store();
+    launch_build_sort_keys(ns, pctl.p, nn, skey[0].p, sval[0].p, nsel.p, stream);
+    unsigned long long nl = 0;
+    d2h(&nl, nsel.p, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
     ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
-    zc.reserve(2);
-    CPG_HIP(hipMemsetAsync(zc.p, 0, 2 * sizeof(unsigned long long), stream));
-    launch_build_sort_keys(live_key[cur].p, live_idx[cur].p, nl, n_level.p, skey[0].p, sval[0].p,
-                           zc.p, stream);
     SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
     const int bytes[6] = {0, 1, 4, 5, 6, 7};
     const int res = radix_sort_pairs(sb, nl, bytes, 6, stream);
-    unsigned long long z[2] = {0, 0};
-    d2h(z, zc.p, 2, stream);
-    CPG_HIP(hipStreamSynchronize(stream));
-    const uint64_t avail = nl - z[1];
-    const uint64_t nout = std::min<uint64_t>(static_cast<uint64_t>(k), avail);
+    const uint64_t nout = std::min<uint64_t>(static_cast<uint64_t>(k), nl);
     r_cost.reserve(std::max<uint64_t>(1, nout));
     r_level.reserve(std::max<uint64_t>(1, nout)); r_from.reserve(std::max<uint64_t>(1, nout));
     r_to.reserve(std::max<uint64_t>(1, nout)); r_pidx.reserve(std::max<uint64_t>(1, nout));
     r_parent.reserve(std::max<uint64_t>(1, nout));
-    rank_of.reserve(std::max<uint64_t>(1, c.node_n));
-    launch_fill_int(rank_of.p, -1, c.node_n, stream);
-    NodeStore ns{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, node_cap};
-    launch_emit_report(skey[res].p, sval[res].p, nout, z[0], z[1], ns, r_cost.p, r_level.p,
-                       r_from.p, r_to.p, r_pidx.p, rank_of.p, stream);
+    rank_of.reserve(std::max<uint64_t>(1, nn));
+    launch_fill_int(rank_of.p, -1, nn, stream);
+    launch_emit_report(sval[res].p, nout, ns, r_cost.p, r_level.p, r_from.p, r_to.p, r_pidx.p,
+                       rank_of.p, stream);
     launch_link_parents(r_pidx.p, rank_of.p, nout, r_parent.p, stream);
     CPG_HIP(hipStreamSynchronize(stream));
     const double te = now_ms();

@@ -56,17 +56,16 @@ void launch_sfxt_index(const SfxtArgs& args, int lift_levels, hipStream_t stream
 constexpr int kSfxtLdsSort = 2048;  // longest deviation list sorted in LDS
 
 // ---------------- PFXT ----------------
+constexpr int kHistParts = 256;  // per-block partial histograms per pass
+
 struct PfxtCtl {
-  unsigned long long live_n;       // entries in the current live buffer
-  unsigned long long live_next_n;  // entries written by compaction
   unsigned long long batch_n;      // parents selected this round
-  unsigned long long node_n;       // prefix-tree nodes stored
+  unsigned long long node_n;       // prefix-tree nodes stored (may pass node_cap on overflow)
   unsigned long long child_n;      // count pass: children <= bound
   unsigned long long expanded;     // parents expanded, all rounds
   unsigned long long hops;         // chain vertices visited (stats)
   unsigned long long devvis;       // deviation records examined (stats)
-  unsigned long long fill_node_base;  // node_n before the fill pass
-  unsigned long long fill_live_base;  // live_n before the fill pass
+  unsigned long long fill_node_base;  // node_n when the round began
   unsigned long long node_cap;     // node store capacity
   unsigned ukey;                   // bound: K-th smallest nonzero cost key (kKeyInf = none)
   unsigned thkey;                  // batch threshold key
@@ -75,17 +74,18 @@ struct PfxtCtl {
   int done;                        // no parent left to expand
   int overflow;                    // node capacity exceeded in a fill pass
   int rounds;
-  int swap_pending;                // compaction output becomes the live buffer
   int err;                         // a walk exceeded its step bound (corrupt index)
-  int pad;
 };
 
+// Prefix-tree node store: one slot per generated path, roots first.
 struct NodeStore {
   float* cost;
-  int* to;
-  int* from;
+  int* to;        // head of the path's suffix chain
+  int* from;      // deviation vertex (-1 for a root)
   int* parent;
   int* level;
+  uint32_t* key;  // orderable cost key
+  uint8_t* flag;  // 1 once expanded
   uint64_t cap;
 };
 
@@ -95,44 +95,32 @@ struct WalkArgs {
   const int4* devs;
   const uint32_t* batch;
   NodeStore nodes;
-  uint32_t* live_key;     // live buffer receiving appended children
-  uint32_t* live_idx;
   unsigned* hist;         // count pass histogram (kHistBins)
   PfxtCtl* ctl;
   float max_abs_distf;
   float max_abs_wgt;
   int collect_stats;
-  int hist_fill;          // fill pass also histograms the children (no count pass this round)
   int step_limit;         // bound on one walk's steps: (levels + 2) * (max out-degree + 2)
 };
 
-struct LiveArgs {
-  const uint32_t* key;
-  uint32_t* idx;          // expanded flag is set in place
-  uint32_t* out_key;      // compaction target
-  uint32_t* out_idx;
-  uint32_t* batch;
-  unsigned* hist_all;
-  unsigned* hist_pend;
-  PfxtCtl* ctl;
-};
-
 void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dist,
-                  const float* distf, NodeStore nodes, uint32_t* live_key, uint32_t* live_idx,
-                  PfxtCtl* ctl, hipStream_t s);
-void launch_round_begin(PfxtCtl* ctl, hipStream_t s);
+                  const float* distf, NodeStore nodes, PfxtCtl* ctl, hipStream_t s);
 void launch_probe_setup(PfxtCtl* ctl, uint32_t ukey, uint32_t shift, unsigned long long nb,
                         hipStream_t s);
 void launch_iota(uint32_t* p, uint64_t n, int grid, hipStream_t s);
 void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s);
-void launch_hist_live(const LiveArgs& a, int grid, hipStream_t s);
+// Histograms of stored paths <= U (mode 0: all, 1: this round's children)
+// into hist_all (nonzero keys) and, with_pend, hist_pend (unexpanded).
+void launch_hist_nodes(const NodeStore& nodes, PfxtCtl* ctl, int mode, bool with_pend,
+                       unsigned* part_all, unsigned* part_pend, unsigned* hist_all,
+                       unsigned* hist_pend, bool accumulate, hipStream_t s);
 void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long batch_target,
                          hipStream_t s);
-void launch_make_batch(const LiveArgs& a, int grid, hipStream_t s);
+void launch_make_batch(const NodeStore& nodes, PfxtCtl* ctl, uint32_t* batch, int grid,
+                       hipStream_t s);
 void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s);
 void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
                          hipStream_t s);
-void launch_compact_live(const LiveArgs& a, int grid, hipStream_t s);
 
 // ---------------- final selection ----------------
 // Stable LSD radix sort of (key, value) pairs over the given key bytes.
@@ -145,12 +133,12 @@ struct SortBufs {
 int radix_sort_pairs(SortBufs& b, uint64_t n, const int* bytes, int nbytes, hipStream_t s);
 uint64_t sort_tiles(uint64_t n);
 
-void launch_build_sort_keys(const uint32_t* live_key, const uint32_t* live_idx, uint64_t n,
-                            const int* node_level, uint64_t* key, uint32_t* val,
-                            unsigned long long* zero_counts, hipStream_t s);
-void launch_emit_report(const uint64_t* key, const uint32_t* val, uint64_t nout, uint64_t nneg,
-                        uint64_t nzero, NodeStore nodes, float* ocost, int* olevel, int* ofrom,
-                        int* oto, int* opidx, int* rank_of, hipStream_t s);
+// Sort input: stored paths with a nonzero key <= U (nmax = stored count).
+void launch_build_sort_keys(const NodeStore& nodes, const PfxtCtl* ctl, uint64_t nmax,
+                            uint64_t* key, uint32_t* val, unsigned long long* count, hipStream_t s);
+void launch_emit_report(const uint32_t* val, uint64_t nout, NodeStore nodes, float* ocost,
+                        int* olevel, int* ofrom, int* oto, int* opidx, int* rank_of,
+                        hipStream_t s);
 void launch_link_parents(const int* opidx, const int* rank_of, uint64_t nout, int* oparent,
                          hipStream_t s);
 void launch_fill_int(int* p, int v, uint64_t n, hipStream_t s);

@@ -9,23 +9,26 @@
 // (gpucpg/gpucpg.cu:2670-2676, :3271-3276).
 //
 // Instead of the reference's short/long piles and split schedule
-// (gpucpg/gpucpg.cu:19134-21228), each round
-//   1. picks the cheapest unexpanded live paths (histogram select),
-//   2. walks their suffix chains once to histogram the children's costs
-//      (count pass),
-//   3. tightens the bound U to the K-th smallest nonzero cost among the
-//      live paths and those children,
-//   4. walks again and stores only children with cost <= U (fill pass),
-//   5. drops live paths above U.
-// U is always the K-th smallest cost of K distinct generated paths, so no
-// path of the final top K is ever dropped while child cost >= parent cost.
-// The walk skips chain segments via the deviation index built by the SFXT
-// kernel: a vertex whose smallest deviation delta exceeds
-// X = (U - parent) + slack is skipped through its next-smaller pointer and
-// its sorted deviation list is cut at the first delta > X.  slack bounds
-// the rounding gap between delta and the binary32 cost recurrence, so the
-// skip never hides a child whose computed cost is <= U; the exact
-// comparison cost <= U decides emission.
+// (gpucpg/gpucpg.cu:19134-21228), every generated path stays in the node
+// store with its orderable cost key and an expanded flag, and each round
+//   1. histograms the stored paths at or below the bound U (all nonzero
+//      ones, and the unexpanded ones),
+//   2. picks the cheapest unexpanded paths as the batch (histogram select),
+//   3. walks their suffix chains and stores every child with cost <= U,
+//   4. tightens U to the K-th smallest nonzero cost among the stored paths
+//      (histogram of the old paths plus the new children).
+// While U is unbounded a count pass histograms the children before the
+// fill pass stores them, so that only paths at or below the tightened U
+// are stored.  U is always the upper end of the histogram bin where the
+// running count of K distinct stored paths is reached, so no path of the
+// final top K lies above it while child cost >= parent cost.
+// The walk skips chain segments via the deviation index built by the SFXT:
+// a vertex whose smallest deviation delta exceeds X = (U - parent) + slack
+// is skipped through its next-smaller pointer and its sorted deviation
+// list is cut at the first delta > X.  slack bounds the rounding gap
+// between delta and the binary32 cost recurrence, so the skip never hides
+// a child whose computed cost is <= U; the exact comparison cost <= U
+// decides emission.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -82,8 +85,7 @@ __device__ inline void clear_hist(unsigned* h) {
 
 __global__ void __launch_bounds__(kBlock) roots_kernel(const int* srcs, const uint8_t* keep, int nsrc,
                                                        const int* dist, const float* distf,
-                                                       NodeStore nodes, uint32_t* live_key,
-                                                       uint32_t* live_idx, PfxtCtl* ctl) {
+                                                       NodeStore nodes, PfxtCtl* ctl) {
   for (int base = blockIdx.x * blockDim.x; base < nsrc; base += gridDim.x * blockDim.x) {
     const int i = base + threadIdx.x;
     bool take = false;
@@ -93,7 +95,6 @@ __global__ void __launch_bounds__(kBlock) roots_kernel(const int* srcs, const ui
       take = keep[i] != 0 && dist[s] != kIntMax;
     }
     const uint64_t slot = wave_reserve(&ctl->node_n, take);
-    const uint64_t lslot = wave_reserve(&ctl->live_n, take);
     if (take) {
       const float c = distf[s];
       nodes.cost[slot] = c;
@@ -101,23 +102,56 @@ __global__ void __launch_bounds__(kBlock) roots_kernel(const int* srcs, const ui
       nodes.from[slot] = -1;
       nodes.parent[slot] = -1;
       nodes.level[slot] = 0;
-      live_key[lslot] = float_key(c);
-      live_idx[lslot] = static_cast<uint32_t>(slot);
+      nodes.key[slot] = float_key(c);
+      nodes.flag[slot] = 0;
     }
   }
 }
 
-__global__ void __launch_bounds__(kBlock) hist_live_kernel(LiveArgs a) {
-  PfxtCtl* ctl = a.ctl;
-  const uint64_t n = ctl->live_n;
-  const uint32_t klo = ctl->klo, shift = ctl->shift;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t k = a.key[i];
-    const uint32_t b = key_bin(k, klo, shift);
-    if (!key_is_zero(k)) atomicAdd(&a.hist_all[b], 1u);
-    if (!(a.idx[i] & kExpandedBit)) atomicAdd(&a.hist_pend[b], 1u);
+// Per-block histograms of the stored paths with key <= U over the window
+// (klo, shift), counted in LDS and written out as one partial per block:
+// all nonzero keys, and (with_pend) the unexpanded paths.  mode 0 covers
+// every stored path, mode 1 this round's children only.
+__global__ void __launch_bounds__(kBlock) hist_nodes_kernel(NodeStore nodes, PfxtCtl* ctl, int mode,
+                                                            int with_pend, unsigned* part_all,
+                                                            unsigned* part_pend) {
+  __shared__ unsigned h_all[kHistBins];
+  __shared__ unsigned h_pend[kHistBins];
+  for (int j = threadIdx.x; j < kHistBins; j += blockDim.x) {
+    h_all[j] = 0;
+    h_pend[j] = 0;
   }
+  __syncthreads();
+  const uint64_t nn = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
+  const uint64_t lo = mode == 0 ? 0 : ctl->fill_node_base;
+  const uint32_t uk = ctl->ukey;
+  const uint32_t klo = ctl->klo, shift = ctl->shift;
+  for (uint64_t i = lo + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nn;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t k = nodes.key[i];
+    if (k > uk) continue;
+    const uint32_t b = key_bin(k, klo, shift);
+    if (!key_is_zero(k)) atomicAdd(&h_all[b], 1u);
+    if (with_pend && nodes.flag[i] == 0) atomicAdd(&h_pend[b], 1u);
+  }
+  __syncthreads();
+  unsigned* pa = part_all + static_cast<size_t>(blockIdx.x) * kHistBins;
+  unsigned* pp = part_pend + static_cast<size_t>(blockIdx.x) * kHistBins;
+  for (int j = threadIdx.x; j < kHistBins; j += blockDim.x) {
+    pa[j] = h_all[j];
+    if (with_pend) pp[j] = h_pend[j];
+  }
+}
+
+// Sums nparts per-block partial histograms into out (added to out when
+// accumulate, written over it otherwise).
+__global__ void __launch_bounds__(kBlock) reduce_hist_kernel(const unsigned* part, int nparts,
+                                                             unsigned* out, int accumulate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= kHistBins) return;
+  unsigned sum = accumulate ? out[j] : 0u;
+  for (int b = 0; b < nparts; ++b) sum += part[static_cast<size_t>(b) * kHistBins + j];
+  out[j] = sum;
 }
 
 __global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, unsigned* hist_pend,
@@ -128,7 +162,6 @@ __global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, u
     ctl->batch_n = 0;
     ctl->child_n = 0;
     ctl->fill_node_base = ctl->node_n;
-    ctl->fill_live_base = ctl->live_n;
   }
   unsigned long long total = 0;
   const uint32_t b = find_crossing(hist_pend, target, &total);
@@ -142,34 +175,27 @@ __global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, u
       ctl->thkey = up < ctl->ukey ? up : ctl->ukey;
     }
   }
-  __syncthreads();
-  clear_hist(hist_pend);
 }
 
 constexpr int kTileItems = 8;  // items per thread per tile in the filter kernels
 
-// Marks the live paths with key <= thkey that are not expanded yet and
+// Marks the stored paths with key <= thkey that are not expanded yet and
 // appends them to the batch; one counter atomic per block and tile.
-__global__ void __launch_bounds__(kBlock) make_batch_kernel(LiveArgs a) {
-  PfxtCtl* ctl = a.ctl;
+__global__ void __launch_bounds__(kBlock) make_batch_kernel(NodeStore nodes, PfxtCtl* ctl,
+                                                            uint32_t* batch) {
   if (ctl->done) return;
-  const uint64_t n = ctl->live_n;
+  const uint64_t n = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
   const uint32_t th = ctl->thkey;
   constexpr uint64_t kTile = static_cast<uint64_t>(kBlock) * kTileItems;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile; base < n;
        base += static_cast<uint64_t>(gridDim.x) * kTile) {
-    uint32_t id[kTileItems];
     unsigned mask = 0, cnt = 0;
 #pragma unroll
     for (int q = 0; q < kTileItems; ++q) {
       const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
-      id[q] = 0;
-      if (i < n) {
-        id[q] = a.idx[i];
-        if (!(id[q] & kExpandedBit) && a.key[i] <= th) {
-          mask |= 1u << q;
-          ++cnt;
-        }
+      if (i < n && nodes.flag[i] == 0 && nodes.key[i] <= th) {
+        mask |= 1u << q;
+        ++cnt;
       }
     }
     uint64_t slot = block_reserve<kBlock>(&ctl->batch_n, cnt);
@@ -177,8 +203,8 @@ __global__ void __launch_bounds__(kBlock) make_batch_kernel(LiveArgs a) {
     for (int q = 0; q < kTileItems; ++q) {
       if (mask & (1u << q)) {
         const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
-        a.idx[i] = id[q] | kExpandedBit;
-        a.batch[slot++] = id[q];
+        nodes.flag[i] = 1;
+        batch[slot++] = static_cast<uint32_t>(i);
       }
     }
   }
@@ -188,10 +214,8 @@ constexpr int kStage = 256;  // per-wave LDS slots for children awaiting a flush
 
 // Walks the suffix chains of the batch's parents and finds every child
 // with cost <= U.  Count pass (kFill = false): histogram the children's
-// keys.  Fill pass: store them (node store + live buffer, after the live
-// entries present when the round began); with a.hist_fill it also
-// histograms them, for rounds that skip the count pass.  Children are
-// staged per wave in LDS and written with one counter atomic per flush.
+// keys.  Fill pass: append them to the node store, unexpanded.  Children
+// are staged per wave in LDS and written with one counter atomic per flush.
 template <bool kFill>
 __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
   constexpr int kWaves = kBlock / 64;
@@ -211,8 +235,6 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
   const float U = key_float(ukey);
   const uint32_t klo = ctl->klo, shift = ctl->shift;
   const uint64_t cap = a.nodes.cap;
-  const uint64_t node_base = kFill ? ctl->fill_node_base : 0;
-  const uint64_t live_base = kFill ? ctl->fill_live_base : 0;
   const float mag = 2.0f * a.max_abs_distf + a.max_abs_wgt + 1.0f;
   unsigned long long hops = 0, devv = 0, counted = 0;
   int staged = 0;  // wave-uniform
@@ -234,11 +256,8 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
         a.nodes.from[slot] = s_from[wave][q];
         a.nodes.parent[slot] = s_parent[wave][q];
         a.nodes.level[slot] = s_level[wave][q];
-        const uint64_t ls = live_base + (slot - node_base);
-        const uint32_t k = float_key(cost);
-        a.live_key[ls] = k;
-        a.live_idx[ls] = static_cast<uint32_t>(slot);
-        if (a.hist_fill && !key_is_zero(k)) atomicAdd(&a.hist[key_bin(k, klo, shift)], 1u);
+        a.nodes.key[slot] = float_key(cost);
+        a.nodes.flag[slot] = 0;
       } else {
         atomicExch(&ctl->overflow, 1);
       }
@@ -362,57 +381,11 @@ __global__ void __launch_bounds__(kSelBlock) select_bound_kernel(PfxtCtl* ctl, u
     }
     ctl->ukey = uk;
     ctl->shift = window_shift(ctl->klo, uk);
-    ctl->swap_pending = 1;
     ctl->expanded += ctl->batch_n;
     ctl->rounds += 1;
   }
   __syncthreads();
   clear_hist(hist_all);
-}
-
-// Keeps the live paths (old entries and this round's children) with key
-// <= U and rebuilds both histograms for the next round.  Survivors keep
-// their order within a block's tile; tiles land in reservation order.
-__global__ void __launch_bounds__(kBlock) compact_live_kernel(LiveArgs a) {
-  PfxtCtl* ctl = a.ctl;
-  if (ctl->done) return;
-  const uint64_t cap_nodes = ctl->node_cap;
-  const uint64_t nn = ctl->node_n < cap_nodes ? ctl->node_n : cap_nodes;
-  const uint64_t n = ctl->fill_live_base + (nn - ctl->fill_node_base);
-  const uint32_t uk = ctl->ukey;
-  const uint32_t klo = ctl->klo, shift = ctl->shift;
-  constexpr uint64_t kTile = static_cast<uint64_t>(kBlock) * kTileItems;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile; base < n;
-       base += static_cast<uint64_t>(gridDim.x) * kTile) {
-    uint32_t k[kTileItems], id[kTileItems];
-    unsigned mask = 0, cnt = 0;
-#pragma unroll
-    for (int q = 0; q < kTileItems; ++q) {
-      const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
-      k[q] = 0;
-      id[q] = 0;
-      if (i < n) {
-        k[q] = a.key[i];
-        id[q] = a.idx[i];
-        if (k[q] <= uk) {
-          mask |= 1u << q;
-          ++cnt;
-        }
-      }
-    }
-    uint64_t slot = block_reserve<kBlock>(&ctl->live_next_n, cnt);
-#pragma unroll
-    for (int q = 0; q < kTileItems; ++q) {
-      if (mask & (1u << q)) {
-        a.out_key[slot] = k[q];
-        a.out_idx[slot] = id[q];
-        ++slot;
-        const uint32_t b = key_bin(k[q], klo, shift);
-        if (!key_is_zero(k[q])) atomicAdd(&a.hist_all[b], 1u);
-        if (!(id[q] & kExpandedBit)) atomicAdd(&a.hist_pend[b], 1u);
-      }
-    }
-  }
 }
 
 // Bound probe: the walk's inputs for one trial bound ukey over the roots
@@ -454,24 +427,14 @@ __global__ void __launch_bounds__(kSelBlock) probe_finish_kernel(PfxtCtl* ctl, u
   clear_hist(hist_all);
 }
 
-__global__ void round_begin_kernel(PfxtCtl* ctl) {
-  if (ctl->done) return;
-  if (ctl->swap_pending) {
-    ctl->live_n = ctl->live_next_n;
-    ctl->live_next_n = 0;
-    ctl->swap_pending = 0;
-  }
-}
-
 }  // namespace
 
 void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dist,
-                  const float* distf, NodeStore nodes, uint32_t* live_key, uint32_t* live_idx,
-                  PfxtCtl* ctl, hipStream_t s) {
+                  const float* distf, NodeStore nodes, PfxtCtl* ctl, hipStream_t s) {
   if (nsrc <= 0) return;
   const int grid = (nsrc + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(roots_kernel, dim3(grid), dim3(kBlock), 0, s, srcs, keep, nsrc, dist, distf, nodes, live_key, live_idx,
-                                       ctl);
+  hipLaunchKernelGGL(roots_kernel, dim3(grid), dim3(kBlock), 0, s, srcs, keep, nsrc, dist, distf,
+                     nodes, ctl);
   CPG_HIP(hipGetLastError());
 }
 
@@ -492,14 +455,21 @@ void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
   CPG_HIP(hipGetLastError());
 }
 
-void launch_round_begin(PfxtCtl* ctl, hipStream_t s) {
-  hipLaunchKernelGGL(round_begin_kernel, dim3(1), dim3(1), 0, s, ctl);
+void launch_hist_nodes(const NodeStore& nodes, PfxtCtl* ctl, int mode, bool with_pend,
+                       unsigned* part_all, unsigned* part_pend, unsigned* hist_all,
+                       unsigned* hist_pend, bool accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(hist_nodes_kernel, dim3(kHistParts), dim3(kBlock), 0, s, nodes, ctl, mode,
+                     with_pend ? 1 : 0, part_all, part_pend);
   CPG_HIP(hipGetLastError());
-}
-
-void launch_hist_live(const LiveArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(hist_live_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+  const int rg = (kHistBins + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(reduce_hist_kernel, dim3(rg), dim3(kBlock), 0, s, part_all, kHistParts,
+                     hist_all, accumulate ? 1 : 0);
   CPG_HIP(hipGetLastError());
+  if (with_pend) {
+    hipLaunchKernelGGL(reduce_hist_kernel, dim3(rg), dim3(kBlock), 0, s, part_pend, kHistParts,
+                       hist_pend, accumulate ? 1 : 0);
+    CPG_HIP(hipGetLastError());
+  }
 }
 
 void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long target,
@@ -508,8 +478,9 @@ void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long t
   CPG_HIP(hipGetLastError());
 }
 
-void launch_make_batch(const LiveArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(make_batch_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+void launch_make_batch(const NodeStore& nodes, PfxtCtl* ctl, uint32_t* batch, int grid,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(make_batch_kernel, dim3(grid), dim3(kBlock), 0, s, nodes, ctl, batch);
   CPG_HIP(hipGetLastError());
 }
 
@@ -521,11 +492,6 @@ void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s) {
 
 void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s) {
   hipLaunchKernelGGL(select_bound_kernel, dim3(1), dim3(kSelBlock), 0, s, ctl, hist_all, k);
-  CPG_HIP(hipGetLastError());
-}
-
-void launch_compact_live(const LiveArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(compact_live_kernel, dim3(grid), dim3(kBlock), 0, s, a);
   CPG_HIP(hipGetLastError());
 }
 

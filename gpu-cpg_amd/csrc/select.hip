@@ -3,11 +3,10 @@
 //
 // The reference sorts its whole short pile with thrust::sort on slack
 // (gpucpg/gpucpg.cu:21323-21331), erases zero-cost entries (:21338-21358)
-// and returns the first K (get_slacks :21409-21420).  Here the live set
-// already holds only paths at or below the K-th bound, the sort key is
-// (orderable cost bits, prefix-tree level) so that a parent always ranks
-// before a child of equal cost, and zero-cost paths are skipped while
-// ranking.
+// and returns the first K (get_slacks :21409-21420).  Here the sort input
+// is every stored path with a nonzero cost at or below the K-th bound, and
+// the sort key is (orderable cost bits, prefix-tree level) so that a
+// parent always ranks before a child of equal cost.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -36,27 +35,48 @@ __global__ void __launch_bounds__(kSortBlock) sort_hist_kernel(const uint64_t* k
   ghist[static_cast<uint64_t>(threadIdx.x) * ntiles + t] = h[threadIdx.x];
 }
 
-// Exclusive scan of ghist[0..len) in place, one block.
+// Exclusive scan of ghist[0..len) in place, one block: chunks of
+// kScanBlock * 4 counts, each thread taking four consecutive ones, a wave
+// scan by shuffles, a scan of the wave totals, and a carry across chunks.
 __global__ void __launch_bounds__(kScanBlock) sort_scan_kernel(unsigned* ghist, uint64_t len) {
-  __shared__ unsigned long long part[kScanBlock];
-  const uint64_t per = (len + kScanBlock - 1) / kScanBlock;
-  const uint64_t b = threadIdx.x * per;
-  const uint64_t e = b + per < len ? b + per : len;
-  unsigned long long sum = 0;
-  for (uint64_t i = b; i < e; ++i) sum += ghist[i];
-  part[threadIdx.x] = sum;
+  constexpr int kW = kScanBlock / 64;
+  __shared__ unsigned s_wave[kW];
+  __shared__ unsigned long long s_carry;
+  const int t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
+  if (t == 0) s_carry = 0;
   __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    const unsigned long long v = threadIdx.x >= static_cast<unsigned>(off) ? part[threadIdx.x - off] : 0ull;
+  for (uint64_t c = 0; c < len; c += static_cast<uint64_t>(kScanBlock) * 4) {
+    const uint64_t i0 = c + static_cast<uint64_t>(t) * 4;
+    unsigned v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = i0 + q < len ? ghist[i0 + q] : 0u;
+    const unsigned mine = v[0] + v[1] + v[2] + v[3];
+    unsigned incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned x = __shfl_up(incl, o);
+      if (lane >= o) incl += x;
+    }
+    if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
-    part[threadIdx.x] += v;
+    if (wave == 0) {
+      unsigned w = lane < kW ? s_wave[lane] : 0u;
+      for (int o = 1; o < kW; o <<= 1) {
+        const unsigned x = __shfl_up(w, o);
+        if (lane >= o) w += x;
+      }
+      if (lane < kW) s_wave[lane] = w;  // inclusive wave totals
+    }
     __syncthreads();
-  }
-  unsigned long long run = part[threadIdx.x] - sum;
-  for (uint64_t i = b; i < e; ++i) {
-    const unsigned c = ghist[i];
-    ghist[i] = static_cast<unsigned>(run);
-    run += c;
+    const unsigned long long carry = s_carry;
+    unsigned long long run = carry + (wave ? s_wave[wave - 1] : 0u) + (incl - mine);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (i0 + q < len) ghist[i0 + q] = static_cast<unsigned>(run);
+      run += v[q];
+    }
+    __syncthreads();
+    if (t == kScanBlock - 1) s_carry = carry + s_wave[kW - 1];
+    __syncthreads();
   }
 }
 
@@ -111,37 +131,46 @@ __global__ void __launch_bounds__(kSortBlock) sort_scatter_kernel(const uint64_t
   }
 }
 
-__global__ void build_keys_kernel(const uint32_t* live_key, const uint32_t* live_idx, uint64_t n,
-                                  const int* node_level, uint64_t* key, uint32_t* val,
-                                  unsigned long long* zc) {
-  unsigned long long neg = 0, zero = 0;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t k = live_key[i];
-    const uint32_t id = live_idx[i] & ~kExpandedBit;
-    const uint32_t lvl = static_cast<uint32_t>(node_level[id]);
-    key[i] = (static_cast<uint64_t>(k) << 32) | lvl;
-    val[i] = id;
-    if (key_is_zero(k)) ++zero;
-    else if (k < kKeyNegZero) ++neg;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    neg += __shfl_xor(neg, o);
-    zero += __shfl_xor(zero, o);
-  }
-  if (__lane_id() == 0) {
-    if (neg) atomicAdd(&zc[0], neg);
-    if (zero) atomicAdd(&zc[1], zero);
+// Sort input: the stored paths with a nonzero key <= U as
+// (key << 32 | level) -> node index; *count receives their number.
+__global__ void __launch_bounds__(256) build_keys_kernel(NodeStore nodes, const PfxtCtl* ctl,
+                                                         uint64_t* key, uint32_t* val,
+                                                         unsigned long long* count) {
+  const uint64_t n = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
+  const uint32_t uk = ctl->ukey;
+  constexpr int kItems8 = 8;
+  constexpr uint64_t kT = 256ull * kItems8;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kT; base < n;
+       base += static_cast<uint64_t>(gridDim.x) * kT) {
+    unsigned mask = 0, cnt = 0;
+    uint32_t k[kItems8];
+#pragma unroll
+    for (int q = 0; q < kItems8; ++q) {
+      const uint64_t i = base + static_cast<uint64_t>(q) * 256 + threadIdx.x;
+      k[q] = i < n ? nodes.key[i] : 0u;
+      if (i < n && k[q] <= uk && !key_is_zero(k[q])) {
+        mask |= 1u << q;
+        ++cnt;
+      }
+    }
+    uint64_t slot = block_reserve<256>(count, cnt);
+#pragma unroll
+    for (int q = 0; q < kItems8; ++q) {
+      if (mask & (1u << q)) {
+        const uint64_t i = base + static_cast<uint64_t>(q) * 256 + threadIdx.x;
+        key[slot] = (static_cast<uint64_t>(k[q]) << 32) | static_cast<uint32_t>(nodes.level[i]);
+        val[slot] = static_cast<uint32_t>(i);
+        ++slot;
+      }
+    }
   }
 }
 
-__global__ void emit_report_kernel(const uint32_t* val, uint64_t nout, uint64_t nneg, uint64_t nzero,
-                                   NodeStore nodes, float* ocost, int* olevel, int* ofrom, int* oto,
-                                   int* opidx, int* rank_of) {
+__global__ void emit_report_kernel(const uint32_t* val, uint64_t nout, NodeStore nodes, float* ocost,
+                                   int* olevel, int* ofrom, int* oto, int* opidx, int* rank_of) {
   for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < nout;
        r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t pos = r < nneg ? r : r + nzero;
-    const uint32_t id = val[pos];
+    const uint32_t id = val[r];
     ocost[r] = nodes.cost[id];
     olevel[r] = nodes.level[id];
     ofrom[r] = nodes.from[id];
@@ -192,21 +221,20 @@ int radix_sort_pairs(SortBufs& b, uint64_t n, const int* bytes, int nbytes, hipS
   return cur;
 }
 
-void launch_build_sort_keys(const uint32_t* live_key, const uint32_t* live_idx, uint64_t n,
-                            const int* node_level, uint64_t* key, uint32_t* val,
-                            unsigned long long* zero_counts, hipStream_t s) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(build_keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, live_key, live_idx, n, node_level, key, val,
-                                                     zero_counts);
+void launch_build_sort_keys(const NodeStore& nodes, const PfxtCtl* ctl, uint64_t nmax,
+                            uint64_t* key, uint32_t* val, unsigned long long* count, hipStream_t s) {
+  if (nmax == 0) return;
+  hipLaunchKernelGGL(build_keys_kernel, dim3(grid_for((nmax + 7) / 8, 256)), dim3(256), 0, s,
+                     nodes, ctl, key, val, count);
   CPG_HIP(hipGetLastError());
 }
 
-void launch_emit_report(const uint64_t* /*key*/, const uint32_t* val, uint64_t nout, uint64_t nneg,
-                        uint64_t nzero, NodeStore nodes, float* ocost, int* olevel, int* ofrom,
-                        int* oto, int* opidx, int* rank_of, hipStream_t s) {
+void launch_emit_report(const uint32_t* val, uint64_t nout, NodeStore nodes, float* ocost,
+                        int* olevel, int* ofrom, int* oto, int* opidx, int* rank_of,
+                        hipStream_t s) {
   if (nout == 0) return;
-  hipLaunchKernelGGL(emit_report_kernel, dim3(grid_for(nout, 256)), dim3(256), 0, s, val, nout, nneg, nzero, nodes, ocost,
-                                                         olevel, ofrom, oto, opidx, rank_of);
+  hipLaunchKernelGGL(emit_report_kernel, dim3(grid_for(nout, 256)), dim3(256), 0, s, val, nout,
+                     nodes, ocost, olevel, ofrom, oto, opidx, rank_of);
   CPG_HIP(hipGetLastError());
 }
 

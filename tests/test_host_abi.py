@@ -154,3 +154,44 @@ def test_unit_weight_variant_keeps_edges():
     n1, u1, v1, w1 = cpgamd.gen_preset(1, 0, True)
     assert n == n1 and np.array_equal(u, u1) and np.array_equal(v, v1)
     assert np.all(w1 == 1.0)
+
+
+FACADE_DRIVER = r"""
+#include <cstdio>
+#include "gpucpg_amd.hpp"
+int main(int argc, char** argv) {
+  gpucpg::CpGen g;
+  g.read_input(argv[1]);
+  std::printf("%zu %zu\n", g.num_verts(), g.num_edges());
+  try {
+    g.read_input(std::string(argv[1]) + ".missing");
+  } catch (const std::runtime_error& e) {
+    std::printf("error %s\n", e.what());
+  }
+  return 0;
+}
+"""
+
+
+def test_cpp_facade_compiles_links_and_reads_input(tmp_path, golden_dir):
+    """include/gpucpg_amd.hpp: a reference-style driver builds with g++
+    against libcpgamd.so and reads a benchmark file (host calls only)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "gpu-cpg_amd")
+    src = tmp_path / "driver.cpp"
+    src.write_text(FACADE_DRIVER)
+    exe = tmp_path / "driver"
+    subprocess.run([gxx, "-std=c++17", "-O1", "-I", os.path.join(root, "include"), str(src),
+                    "-L", libdir, "-lcpgamd", "-Wl,-rpath," + libdir, "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe), os.path.join(golden_dir, "simple.txt")], check=True,
+                         capture_output=True, text=True).stdout.split("\n")
+    og = pyoracle.OracleGraph.from_text(os.path.join(golden_dir, "simple.txt"))
+    n, m = og.n, og.m
+    assert out[0] == f"{n} {m}"
+    assert out[1].startswith("error ")
