@@ -282,11 +282,11 @@ struct cpg_handle {
     if (cap > 0x7fffffffull) throw std::runtime_error("node capacity exceeds 2^31");
     n_cost.reserve(cap); n_to.reserve(cap); n_from.reserve(cap);
     n_parent.reserve(cap); n_level.reserve(cap);
-    n_key.reserve(cap); n_flag.reserve(cap);
+    n_key.reserve(cap + 16); n_flag.reserve(cap + 16);  // vector loads read whole groups of 8
     batch.reserve(cap);
-    hist_all.reserve(kHistBins);
+    hist_all.reserve(2 * kHistBins);
     hist_pend.reserve(kHistBins);
-    hist_part.reserve(2 * static_cast<size_t>(kHistParts) * kHistBins);
+    hist_part.reserve(3 * static_cast<size_t>(kHistParts) * kHistBins);
     pctl.reserve(1);
     node_cap = cap;
   }
@@ -305,9 +305,8 @@ struct cpg_handle {
 
   // Histograms of the stored paths at or below the bound (see launch_hist_nodes).
   void hist_nodes(int mode, bool with_pend, bool accumulate) {
-    launch_hist_nodes(node_store(), pctl.p, mode, with_pend, hist_part.p,
-                      hist_part.p + static_cast<size_t>(kHistParts) * kHistBins, hist_all.p,
-                      hist_pend.p, accumulate, stream);
+    launch_hist_nodes(node_store(), pctl.p, mode, with_pend, hist_part.p, hist_all.p, hist_pend.p,
+                      accumulate, stream);
   }
 
   uint32_t probe_bound(WalkArgs& wa, int64_t k, const std::vector<uint8_t>& kp,
@@ -323,12 +322,13 @@ struct cpg_handle {
     gap = std::max({gap, std::fabs(static_cast<double>(cmin)) * 0x1p-8,
                     static_cast<double>(max_abs_distf) * 0x1p-8, 1e-6});
     uint64_t prev_total = 0;
+    wa.strict = 0;  // a probe counts paths <= the trial bound
     int t = 0;
     for (; t < 8; ++t) {
       const float u0 = static_cast<float>(static_cast<double>(cmin) + gap);
       const uint32_t uk = float_key(u0);
       launch_probe_setup(pctl.p, uk, window_shift(klo, uk), nb, stream);
-      CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * kHistBins, stream));
+      CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * 2 * kHistBins, stream));
       launch_walk(wa, false, grid, stream);
       PfxtCtl pc{};
       d2h(&pc, pctl.p, 1, stream);
@@ -363,7 +363,7 @@ struct cpg_handle {
     }
     timing.probes = std::min(t + 1, 8);
     launch_probe_setup(pctl.p, kKeyInf, window_shift(klo, kKeyInf), 0, stream);
-    CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * kHistBins, stream));
+    CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * 2 * kHistBins, stream));
     return kKeyInf;
   }
 
@@ -462,6 +462,11 @@ struct cpg_handle {
                                        (o.bound_probe == 0 && nkept * (diameter + 1) > (1 << 20)));
       uint32_t ukey_host = kKeyInf;
       if (probe) ukey_host = probe_bound(wa, k, kp, sd, minc, init.klo, trace);
+      // A bound set from stored paths lets a fill pass drop children at the
+      // bound (k stored paths already cost no more).  A bound set from
+      // counted children that are not stored yet (the probe, a count pass)
+      // needs an inclusive fill, or the paths at the bound would be lost.
+      bool bound_from_store = false;
       bool overflow = false;
       for (int r = 0;; ++r) {
         if (r > 100000) throw std::runtime_error("PFXT round limit exceeded");
@@ -476,13 +481,16 @@ struct cpg_handle {
         }
         const bool count_pass = two_pass || ukey_host == kKeyInf;
         if (count_pass) {
+          wa.strict = ukey_host == kKeyInf || bound_from_store ? 1 : 0;
           launch_walk(wa, false, grid, stream);
           if (kernel_timing) CPG_HIP(hipEventRecord(ev_count.b, stream));
           launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
           if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.a, stream));
+          wa.strict = 0;
           launch_walk(wa, true, grid, stream);
           if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
         } else {
+          wa.strict = bound_from_store ? 1 : 0;
           if (kernel_timing) {
             CPG_HIP(hipEventRecord(ev_count.b, stream));
             CPG_HIP(hipEventRecord(ev_fill.a, stream));
@@ -507,9 +515,10 @@ struct cpg_handle {
                          static_cast<double>(key_float(c.ukey)), cm, fm);
         }
         ukey_host = c.ukey;
+        bound_from_store = true;  // every path counted this round is stored now
         if (c.err) throw std::runtime_error("PFXT walk exceeded its step bound");
         if (c.overflow) { overflow = true; break; }
-        if (c.done) break;  // no unexpanded path at or below the bound
+        if (c.done || c.batch_n == 0) break;  // no unexpanded path below the bound
       }
       if (!overflow) break;
       if (attempt >= 4 || cap >= 0x7fffffffull) throw std::runtime_error("prefix-tree node capacity exceeded");

@@ -95,11 +95,12 @@ struct WalkArgs {
   const int4* devs;
   const uint32_t* batch;
   NodeStore nodes;
-  unsigned* hist;         // count pass histogram (kHistBins)
+  unsigned* hist;         // count pass histogram (2 x kHistBins: counts, per-bin max key)
   PfxtCtl* ctl;
   float max_abs_distf;
   float max_abs_wgt;
   int collect_stats;
+  int strict;             // emit children with cost < U (rounds) or <= U (bound probe)
   int step_limit;         // bound on one walk's steps: (levels + 2) * (max out-degree + 2)
 };
 
@@ -109,11 +110,13 @@ void launch_probe_setup(PfxtCtl* ctl, uint32_t ukey, uint32_t shift, unsigned lo
                         hipStream_t s);
 void launch_iota(uint32_t* p, uint64_t n, int grid, hipStream_t s);
 void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s);
-// Histograms of stored paths <= U (mode 0: all, 1: this round's children)
-// into hist_all (nonzero keys) and, with_pend, hist_pend (unexpanded).
+// Histograms of stored paths (mode 0: all, 1: this round's children) into
+// hist_all (2 x kHistBins: counts of nonzero keys <= U, then each bin's
+// largest key) and, with_pend, hist_pend (unexpanded paths with key < U).
+// part holds 3 x kHistParts x kHistBins per-block partials.
 void launch_hist_nodes(const NodeStore& nodes, PfxtCtl* ctl, int mode, bool with_pend,
-                       unsigned* part_all, unsigned* part_pend, unsigned* hist_all,
-                       unsigned* hist_pend, bool accumulate, hipStream_t s);
+                       unsigned* part, unsigned* hist_all, unsigned* hist_pend, bool accumulate,
+                       hipStream_t s);
 void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long batch_target,
                          hipStream_t s);
 void launch_make_batch(const NodeStore& nodes, PfxtCtl* ctl, uint32_t* batch, int grid,

@@ -79,8 +79,10 @@ __device__ uint32_t find_crossing(const unsigned* h, unsigned long long target,
   return b;
 }
 
+// Clears a bound histogram: kHistBins counts followed by kHistBins per-bin
+// maximum keys.
 __device__ inline void clear_hist(unsigned* h) {
-  for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0;
+  for (int i = threadIdx.x; i < 2 * kHistBins; i += blockDim.x) h[i] = 0;
 }
 
 __global__ void __launch_bounds__(kBlock) roots_kernel(const int* srcs, const uint8_t* keep, int nsrc,
@@ -108,50 +110,92 @@ __global__ void __launch_bounds__(kBlock) roots_kernel(const int* srcs, const ui
   }
 }
 
-// Per-block histograms of the stored paths with key <= U over the window
-// (klo, shift), counted in LDS and written out as one partial per block:
-// all nonzero keys, and (with_pend) the unexpanded paths.  mode 0 covers
-// every stored path, mode 1 this round's children only.
+// Per-block histograms over the window (klo, shift), counted in LDS and
+// written out as one partial per block: the nonzero keys <= U with the
+// largest key of each bin, and (with_pend) the unexpanded paths with key
+// < U (those still to be expanded).  mode 0 covers every stored path,
+// mode 1 this round's children only.  Each thread reads four consecutive
+// keys and flags with one vector load each (the store's arrays are padded
+// to a multiple of four slots).
 __global__ void __launch_bounds__(kBlock) hist_nodes_kernel(NodeStore nodes, PfxtCtl* ctl, int mode,
                                                             int with_pend, unsigned* part_all,
-                                                            unsigned* part_pend) {
+                                                            unsigned* part_pend,
+                                                            unsigned* part_max) {
   __shared__ unsigned h_all[kHistBins];
   __shared__ unsigned h_pend[kHistBins];
+  __shared__ unsigned h_max[kHistBins];
   for (int j = threadIdx.x; j < kHistBins; j += blockDim.x) {
     h_all[j] = 0;
     h_pend[j] = 0;
+    h_max[j] = 0;
   }
   __syncthreads();
   const uint64_t nn = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
   const uint64_t lo = mode == 0 ? 0 : ctl->fill_node_base;
   const uint32_t uk = ctl->ukey;
   const uint32_t klo = ctl->klo, shift = ctl->shift;
-  for (uint64_t i = lo + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nn;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t k = nodes.key[i];
-    if (k > uk) continue;
-    const uint32_t b = key_bin(k, klo, shift);
-    if (!key_is_zero(k)) atomicAdd(&h_all[b], 1u);
-    if (with_pend && nodes.flag[i] == 0) atomicAdd(&h_pend[b], 1u);
+  const uint4* key4 = reinterpret_cast<const uint4*>(nodes.key);
+  const uint32_t* flag4 = reinterpret_cast<const uint32_t*>(nodes.flag);
+  for (uint64_t g = lo / 4 + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       g < (nn + 3) / 4; g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint4 kv = key4[g];
+    const uint32_t fv = flag4[g];
+    const uint32_t ks[4] = {kv.x, kv.y, kv.z, kv.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint64_t i = 4 * g + q;
+      const uint32_t k = ks[q];
+      if (i < lo || i >= nn || k > uk) continue;
+      const uint32_t b = key_bin(k, klo, shift);
+      if (!key_is_zero(k)) {
+        atomicAdd(&h_all[b], 1u);
+        atomicMax(&h_max[b], k);
+      }
+      if (with_pend && k < uk && ((fv >> (8 * q)) & 0xffu) == 0) atomicAdd(&h_pend[b], 1u);
+    }
   }
   __syncthreads();
   unsigned* pa = part_all + static_cast<size_t>(blockIdx.x) * kHistBins;
   unsigned* pp = part_pend + static_cast<size_t>(blockIdx.x) * kHistBins;
+  unsigned* pm = part_max + static_cast<size_t>(blockIdx.x) * kHistBins;
   for (int j = threadIdx.x; j < kHistBins; j += blockDim.x) {
     pa[j] = h_all[j];
+    pm[j] = h_max[j];
     if (with_pend) pp[j] = h_pend[j];
   }
 }
 
-// Sums nparts per-block partial histograms into out (added to out when
-// accumulate, written over it otherwise).
-__global__ void __launch_bounds__(kBlock) reduce_hist_kernel(const unsigned* part, int nparts,
-                                                             unsigned* out, int accumulate) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= kHistBins) return;
-  unsigned sum = accumulate ? out[j] : 0u;
-  for (int b = 0; b < nparts; ++b) sum += part[static_cast<size_t>(b) * kHistBins + j];
-  out[j] = sum;
+// Sums (or, take_max, takes the maximum of) the kHistParts per-block
+// partial histograms into out, combined with out when accumulate and
+// written over it otherwise.  A block of kRedBlock
+// threads covers 64 bins: each of its kRedBlock / 64 groups adds an
+// interleaved share of the partials with all its loads in flight, and the
+// groups' sums meet in LDS.
+constexpr int kRedBlock = 1024;
+constexpr int kRedGroups = kRedBlock / 64;
+constexpr int kRedPer = kHistParts / kRedGroups;
+static_assert(kHistParts % kRedGroups == 0, "partials split evenly over groups");
+
+__global__ void __launch_bounds__(kRedBlock) reduce_hist_kernel(const unsigned* part, unsigned* out,
+                                                                int accumulate, int take_max) {
+  __shared__ unsigned s_sum[kRedGroups][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  unsigned v[kRedPer];
+#pragma unroll
+  for (int q = 0; q < kRedPer; ++q)
+    v[q] = part[static_cast<size_t>(grp + q * kRedGroups) * kHistBins + j];
+  unsigned sum = 0;
+#pragma unroll
+  for (int q = 0; q < kRedPer; ++q) sum = take_max ? max(sum, v[q]) : sum + v[q];
+  s_sum[grp][lane] = sum;
+  __syncthreads();
+  if (grp == 0) {
+    unsigned tot = accumulate ? out[j] : 0u;
+    for (int g2 = 0; g2 < kRedGroups; ++g2)
+      tot = take_max ? max(tot, s_sum[g2][lane]) : tot + s_sum[g2][lane];
+    out[j] = tot;
+  }
 }
 
 __global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, unsigned* hist_pend,
@@ -177,34 +221,43 @@ __global__ void __launch_bounds__(kSelBlock) select_batch_kernel(PfxtCtl* ctl, u
   }
 }
 
-constexpr int kTileItems = 8;  // items per thread per tile in the filter kernels
-
-// Marks the stored paths with key <= thkey that are not expanded yet and
-// appends them to the batch; one counter atomic per block and tile.
+// Marks the stored paths with key <= thkey and < U that are not expanded
+// yet and appends them to the batch; one counter atomic per block and tile.
+// A path at U itself is not expanded: at least K stored paths cost <= U,
+// so its children (cost >= U) cannot change the K smallest costs.  A
+// thread takes eight consecutive slots (two 16-byte key loads, one 8-byte
+// flag load).
 __global__ void __launch_bounds__(kBlock) make_batch_kernel(NodeStore nodes, PfxtCtl* ctl,
                                                             uint32_t* batch) {
   if (ctl->done) return;
   const uint64_t n = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
-  const uint32_t th = ctl->thkey;
-  constexpr uint64_t kTile = static_cast<uint64_t>(kBlock) * kTileItems;
+  const uint32_t th = ctl->thkey, uk = ctl->ukey;
+  constexpr uint64_t kTile = static_cast<uint64_t>(kBlock) * 8;
+  const uint4* key4 = reinterpret_cast<const uint4*>(nodes.key);
+  const uint2* flag8 = reinterpret_cast<const uint2*>(nodes.flag);
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTile; base < n;
        base += static_cast<uint64_t>(gridDim.x) * kTile) {
+    const uint64_t i0 = base + 8ull * threadIdx.x;
     unsigned mask = 0, cnt = 0;
+    if (i0 < n) {
+      const uint4 ka = key4[i0 / 4], kb = key4[i0 / 4 + 1];
+      const uint2 fv = flag8[i0 / 8];
+      const uint32_t ks[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
 #pragma unroll
-    for (int q = 0; q < kTileItems; ++q) {
-      const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
-      if (i < n && nodes.flag[i] == 0 && nodes.key[i] <= th) {
-        mask |= 1u << q;
-        ++cnt;
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t f = ((q < 4 ? fv.x : fv.y) >> (8 * (q & 3))) & 0xffu;
+        if (i0 + q < n && f == 0 && ks[q] <= th && ks[q] < uk) {
+          mask |= 1u << q;
+          ++cnt;
+        }
       }
     }
     uint64_t slot = block_reserve<kBlock>(&ctl->batch_n, cnt);
 #pragma unroll
-    for (int q = 0; q < kTileItems; ++q) {
+    for (int q = 0; q < 8; ++q) {
       if (mask & (1u << q)) {
-        const uint64_t i = base + static_cast<uint64_t>(q) * kBlock + threadIdx.x;
-        nodes.flag[i] = 1;
-        batch[slot++] = static_cast<uint32_t>(i);
+        nodes.flag[i0 + q] = 1;
+        batch[slot++] = static_cast<uint32_t>(i0 + q);
       }
     }
   }
@@ -213,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) make_batch_kernel(NodeStore nodes, Pfx
 constexpr int kStage = 256;  // per-wave LDS slots for children awaiting a flush
 
 // Walks the suffix chains of the batch's parents and finds every child
-// with cost <= U.  Count pass (kFill = false): histogram the children's
+// with cost < U (cost <= U for a bound probe, a.strict == 0).  Count pass (kFill = false): histogram the children's
 // keys.  Fill pass: append them to the node store, unexpanded.  Children
 // are staged per wave in LDS and written with one counter atomic per flush.
 template <bool kFill>
@@ -318,7 +371,7 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
           } else {
             cc = ((c + __int_as_float(d.z)) + __int_as_float(d.y)) - dv;
             ++j;
-            if (cc <= U) {
+            if (a.strict ? cc < U : cc <= U) {
               emit = true;
               cw = d.x;
             }
@@ -347,7 +400,11 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
       } else if (emit) {
         ++counted;
         const uint32_t k = float_key(cc);
-        if (!key_is_zero(k)) atomicAdd(&a.hist[key_bin(k, klo, shift)], 1u);
+        if (!key_is_zero(k)) {
+          const uint32_t b = key_bin(k, klo, shift);
+          atomicAdd(&a.hist[b], 1u);
+          atomicMax(&a.hist[kHistBins + b], k);
+        }
       }
     }
   }
@@ -376,8 +433,8 @@ __global__ void __launch_bounds__(kSelBlock) select_bound_kernel(PfxtCtl* ctl, u
   if (threadIdx.x == 0) {
     uint32_t uk = ctl->ukey;
     if (b < static_cast<uint32_t>(kHistBins)) {
-      const uint32_t up = bin_upper_key(b, ctl->klo, ctl->shift);
-      if (up < uk) uk = up;
+      const uint32_t top = hist_all[kHistBins + b];  // largest key in the crossing bin
+      if (top < uk) uk = top;
     }
     ctl->ukey = uk;
     ctl->shift = window_shift(ctl->klo, uk);
@@ -406,7 +463,7 @@ __global__ void iota_kernel(uint32_t* p, uint64_t n) {
 }
 
 // Bound probe succeeded (at least k nonzero paths with cost <= the trial
-// bound were counted): the bound becomes the upper key of the bin where
+// bound were counted): the bound becomes the largest key of the bin where
 // the running count reaches k, and the probe's batch is dropped.
 __global__ void __launch_bounds__(kSelBlock) probe_finish_kernel(PfxtCtl* ctl, unsigned* hist_all,
                                                                  unsigned long long k) {
@@ -415,8 +472,8 @@ __global__ void __launch_bounds__(kSelBlock) probe_finish_kernel(PfxtCtl* ctl, u
   if (threadIdx.x == 0) {
     uint32_t uk = ctl->ukey;
     if (b < static_cast<uint32_t>(kHistBins)) {
-      const uint32_t up = bin_upper_key(b, ctl->klo, ctl->shift);
-      if (up < uk) uk = up;
+      const uint32_t top = hist_all[kHistBins + b];  // largest key in the crossing bin
+      if (top < uk) uk = top;
     }
     ctl->ukey = uk;
     ctl->shift = window_shift(ctl->klo, uk);
@@ -456,18 +513,25 @@ void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
 }
 
 void launch_hist_nodes(const NodeStore& nodes, PfxtCtl* ctl, int mode, bool with_pend,
-                       unsigned* part_all, unsigned* part_pend, unsigned* hist_all,
-                       unsigned* hist_pend, bool accumulate, hipStream_t s) {
+                       unsigned* part, unsigned* hist_all, unsigned* hist_pend, bool accumulate,
+                       hipStream_t s) {
+  const size_t plane = static_cast<size_t>(kHistParts) * kHistBins;
+  unsigned* part_all = part;
+  unsigned* part_pend = part + plane;
+  unsigned* part_max = part + 2 * plane;
   hipLaunchKernelGGL(hist_nodes_kernel, dim3(kHistParts), dim3(kBlock), 0, s, nodes, ctl, mode,
-                     with_pend ? 1 : 0, part_all, part_pend);
+                     with_pend ? 1 : 0, part_all, part_pend, part_max);
   CPG_HIP(hipGetLastError());
-  const int rg = (kHistBins + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(reduce_hist_kernel, dim3(rg), dim3(kBlock), 0, s, part_all, kHistParts,
-                     hist_all, accumulate ? 1 : 0);
+  const int rg = kHistBins / 64;
+  const int acc = accumulate ? 1 : 0;
+  hipLaunchKernelGGL(reduce_hist_kernel, dim3(rg), dim3(kRedBlock), 0, s, part_all, hist_all, acc, 0);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reduce_hist_kernel, dim3(rg), dim3(kRedBlock), 0, s, part_max,
+                     hist_all + kHistBins, acc, 1);
   CPG_HIP(hipGetLastError());
   if (with_pend) {
-    hipLaunchKernelGGL(reduce_hist_kernel, dim3(rg), dim3(kBlock), 0, s, part_pend, kHistParts,
-                       hist_pend, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(reduce_hist_kernel, dim3(rg), dim3(kRedBlock), 0, s, part_pend, hist_pend,
+                       acc, 0);
     CPG_HIP(hipGetLastError());
   }
 }

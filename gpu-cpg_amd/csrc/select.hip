@@ -132,34 +132,39 @@ __global__ void __launch_bounds__(kSortBlock) sort_scatter_kernel(const uint64_t
 }
 
 // Sort input: the stored paths with a nonzero key <= U as
-// (key << 32 | level) -> node index; *count receives their number.
+// (key << 32 | level) -> node index; *count receives their number.  A
+// thread takes eight consecutive slots (two 16-byte key loads).
 __global__ void __launch_bounds__(256) build_keys_kernel(NodeStore nodes, const PfxtCtl* ctl,
                                                          uint64_t* key, uint32_t* val,
                                                          unsigned long long* count) {
   const uint64_t n = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
   const uint32_t uk = ctl->ukey;
-  constexpr int kItems8 = 8;
-  constexpr uint64_t kT = 256ull * kItems8;
+  constexpr uint64_t kT = 256ull * 8;
+  const uint4* key4 = reinterpret_cast<const uint4*>(nodes.key);
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kT; base < n;
        base += static_cast<uint64_t>(gridDim.x) * kT) {
+    const uint64_t i0 = base + 8ull * threadIdx.x;
     unsigned mask = 0, cnt = 0;
-    uint32_t k[kItems8];
+    uint32_t ks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i0 < n) {
+      const uint4 ka = key4[i0 / 4], kb = key4[i0 / 4 + 1];
+      ks[0] = ka.x; ks[1] = ka.y; ks[2] = ka.z; ks[3] = ka.w;
+      ks[4] = kb.x; ks[5] = kb.y; ks[6] = kb.z; ks[7] = kb.w;
 #pragma unroll
-    for (int q = 0; q < kItems8; ++q) {
-      const uint64_t i = base + static_cast<uint64_t>(q) * 256 + threadIdx.x;
-      k[q] = i < n ? nodes.key[i] : 0u;
-      if (i < n && k[q] <= uk && !key_is_zero(k[q])) {
-        mask |= 1u << q;
-        ++cnt;
+      for (int q = 0; q < 8; ++q) {
+        if (i0 + q < n && ks[q] <= uk && !key_is_zero(ks[q])) {
+          mask |= 1u << q;
+          ++cnt;
+        }
       }
     }
     uint64_t slot = block_reserve<256>(count, cnt);
 #pragma unroll
-    for (int q = 0; q < kItems8; ++q) {
+    for (int q = 0; q < 8; ++q) {
       if (mask & (1u << q)) {
-        const uint64_t i = base + static_cast<uint64_t>(q) * 256 + threadIdx.x;
-        key[slot] = (static_cast<uint64_t>(k[q]) << 32) | static_cast<uint32_t>(nodes.level[i]);
-        val[slot] = static_cast<uint32_t>(i);
+        key[slot] = (static_cast<uint64_t>(ks[q]) << 32) |
+                    static_cast<uint32_t>(nodes.level[i0 + q]);
+        val[slot] = static_cast<uint32_t>(i0 + q);
         ++slot;
       }
     }

@@ -33,6 +33,8 @@ struct dim3 {
 };
 struct int2 { int x, y; };
 struct int4 { int x, y, z, w; };
+struct uint2 { unsigned x, y; };
+struct uint4 { unsigned x, y, z, w; };
 inline int2 make_int2(int x, int y) { return int2{x, y}; }
 inline int4 make_int4(int x, int y, int z, int w) { return int4{x, y, z, w}; }
 
@@ -60,6 +62,7 @@ void launch(F kernel, dim3 grid, dim3 block, bool cooperative, A... args);
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline unsigned min(unsigned a, unsigned b) { return a < b ? a : b; }
+inline unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
 
 inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
 inline unsigned __float_as_uint(float f) { unsigned i; std::memcpy(&i, &f, 4); return i; }

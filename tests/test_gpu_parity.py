@@ -218,3 +218,56 @@ def test_c1_bound_probe_modes(k, probe):
         assert g.timings()["probes"] == 0
     want, _ = og.topk(dist, succ, k)
     assert_bits_equal(want, g.get_slacks(k), f"C1 K={k} probe={probe}")
+
+
+def assert_multiset_prefixes(want, got, what, prefixes):
+    assert len(got) == len(want), f"{what}: {len(got)} paths, oracle {len(want)}"
+    for kp in prefixes:
+        if kp > len(want):
+            continue
+        np.testing.assert_array_equal(np.sort(got[:kp]), np.sort(want[:kp]),
+                                      err_msg=f"{what} prefix {kp}")
+
+
+@pytest.mark.parametrize("k", [10000, 50000])
+def test_c1_unit_weights_sorted_multiset(k):
+    """Unit weights: massive equal-cost ties; judged on the sorted multiset."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1, unit=True)
+    say(f"case C1 unit weights K={k}")
+    g = engine_for(n, u, v, w)
+    g.report_paths(k)
+    want, _ = og.topk(dist, succ, k)
+    assert_multiset_prefixes(want, g.get_slacks(k), f"C1 unit K={k}", (100, 1000, k))
+
+
+def test_c3_densified_topk_prefixes_bit_exact():
+    """The 1.5M-vertex DAG densified to mean out-degree 20 (30M edges), K=1M."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(3)
+    say("case C3 sfxt + oracle K=1M")
+    want, st = og.topk(dist, succ, 1_000_000)
+    assert st["nonmonotone"] == 0
+    g = engine_for(n, u, v, w)
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    say("case C3 GPU K=1M")
+    g.report_paths(1_000_000)
+    got = g.get_slacks(1_000_000)
+    say(f"C3 timings {g.timings()}")
+    for kp in (1000, 10_000, 50_000, 100_000, 1_000_000):
+        assert_bits_equal(want[:kp], got[:kp], f"C3 prefix {kp}")
+
+
+def test_c3_unit_weights_sorted_multiset_prefixes():
+    """C3 with every delay 1.0f: equal-cost ties everywhere; sorted multiset
+    per K prefix."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(3, unit=True)
+    say("case C3 unit oracle K=1M")
+    want, _ = og.topk(dist, succ, 1_000_000)
+    g = engine_for(n, u, v, w)
+    say("case C3 unit GPU K=1M")
+    g.report_paths(1_000_000)
+    got = g.get_slacks(1_000_000)
+    say(f"C3 unit timings {g.timings()}")
+    assert_multiset_prefixes(want, got, "C3 unit", (1000, 10_000, 50_000, 100_000, 1_000_000))
