@@ -9,11 +9,15 @@ top-K selection, results left in HBM.  The static cache is off, so every
 step recomputes the SFXT.
 
 Prints one JSON line.  Besides the contract fields it carries
-  roofline      the dominant kernel's achieved algorithmic HBM bandwidth,
-                from HIP events on the engine's stream, against 8 TB/s
-  cpu_baseline  the oracle's host SFXT + best-first PFXT on this host
-                (single thread), same graph and K
-  exact         GPU costs equal the CPU baseline's bit for bit
+  roofline        the dominant kernel's achieved algorithmic HBM bandwidth,
+                  from HIP events on the engine's stream, against 8 TB/s;
+                  traffic = its measured HBM bytes per launch (rocprofv3
+                  FETCH_SIZE + WRITE_SIZE summary committed under profiles/)
+  query_roofline  the whole query against the schedule-independent
+                  algorithmic bytes 52N + 36M + 16 H_K + 12 E_K + 32 C_K + 4 K
+  cpu_baseline    the oracle's host SFXT + best-first PFXT on this host
+                  (single thread), same graph and K
+  exact           GPU costs equal the CPU baseline's bit for bit
 """
 import argparse
 import json
@@ -30,6 +34,32 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by a copy-kernel calibration),
+    or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "traffic_sfxt_levels_walk_c2_k1m.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def load_golden_counts():
+    """H_K/E_K/C_K of the oracle's best-first expansion for this workload."""
+    path = os.path.join(ROOT, "tests", "golden", "synthetic_golden.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["c2"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -81,7 +111,8 @@ def main():
     # kernel-level timing (HIP events on the engine's stream), separate runs
     kt = [run(kernel_timing=True) for _ in range(3)]
     sfxt_kernel_ms = statistics.mean(x["sfxt_kernel_ms"] for x in kt)
-    walk_ms = statistics.mean(x["walk_count_ms"] + x["walk_fill_ms"] for x in kt)
+    levels_ms = statistics.mean(x["sfxt_levels_ms"] for x in kt)
+    walk_ms = statistics.mean(x["walk_fill_ms"] + x["walk_count_ms"] for x in kt)
     walk_launches = kt[0]["walk_launches"]
     # deviation-walk statistics for the algorithmic byte count
     g.reset()
@@ -89,16 +120,16 @@ def main():
     st = g.timings()
 
     # algorithmic bytes per launch
-    #  SFXT kernel: 52 N + 36 M (levelize 20N+12M, relax 16N+12M, successor +
-    #  deviation index 16N+12M), one launch per query
-    sfxt_bytes = 52.0 * n + 36.0 * m
+    #  level loop: levelize 20N+12M, relax 16N+12M, successor + distf 16N+12M
+    #  (52N + 36M), one launch per query
+    levels_bytes = 52.0 * n + 36.0 * m
     #  walk kernels: per visited chain vertex a 16 B vertex record, per
     #  examined deviation a 16 B record, per expanded parent 12 B, per stored
-    #  child 28 B (20 B node + 8 B live entry)
-    walk_bytes = 16.0 * st["hops"] + 16.0 * st["dev_visits"] + 12.0 * 2 * st["expanded"] \
-        + 28.0 * (st["generated"] - st["roots"])
+    #  child 25 B (node slot)
+    walk_bytes = 16.0 * st["hops"] + 16.0 * st["dev_visits"] + 12.0 * st["expanded"] \
+        + 25.0 * (st["generated"] - st["roots"])
     kernels = {
-        "sfxt_kernel": (sfxt_kernel_ms, sfxt_bytes, 1),
+        "sfxt_levels_kernel": (levels_ms, levels_bytes, 1),
         "walk_kernel": (walk_ms, walk_bytes, max(walk_launches, 1)),
     }
     dom = max(kernels, key=lambda name: kernels[name][0])
@@ -107,9 +138,11 @@ def main():
     achieved = (tot_bytes / launches) / (avg_ms * 1e-3) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 3),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "traffic": measured_traffic(dom), "avg_launch_ms": round(avg_ms, 4),
                 "bytes_per_launch": tot_bytes / launches, "launches_per_query": launches}
-
+    # whole query against the schedule-independent algorithmic bytes
+    # B_alg = 52N + 36M + 16 H_K + 12 E_K + 32 C_K + 4 K (oracle counts)
+    gold = load_golden_counts()
     med = statistics.median(x["query_ms"] for x in tims)
     out = {
         "metric": "ms per K=1M exact path query",
@@ -137,9 +170,17 @@ def main():
         "expanded": tims[-1]["expanded"],
         "generated": tims[-1]["generated"],
         "sfxt_kernel_ms": round(sfxt_kernel_ms, 4),
+        "sfxt_levels_ms": round(levels_ms, 4),
         "walk_kernel_ms": round(walk_ms, 4),
+        "probes": tims[-1]["probes"],
         "roofline": roofline,
     }
+    if gold is not None and k == gold["k"]:
+        b_alg = 52.0 * n + 36.0 * m + 16.0 * gold["H_K"] + 12.0 * gold["E_K"] \
+            + 32.0 * gold["C_K"] + 4.0 * k
+        qa = b_alg / (ms_per_step * 1e-3) / 1e9
+        out["query_roofline"] = {"bytes": b_alg, "achieved": round(qa, 3), "unit": "GB/s",
+                                 "frac": round(qa / HBM_PEAK_GBS, 6)}
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -247,8 +247,9 @@ struct cpg_handle {
         ev_sfxt.create();
         ev_levels.create();
         CPG_HIP(hipEventRecord(ev_sfxt.a, stream));
-        CPG_HIP(hipEventRecord(ev_levels.a, stream));
       }
+      launch_sfxt_init(a, stream);
+      if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.a, stream));
       timing.sfxt_blocks = launch_sfxt_levels(a, level_loop_blocks(), stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.b, stream));
       d2h(&c, sctl.p, 1, stream);

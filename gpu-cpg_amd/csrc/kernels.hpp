@@ -49,7 +49,8 @@ struct SfxtArgs {
 
 // Number of lift levels for a graph of num_levels levels (2^levels > chain length + 1).
 int sfxt_lift_levels(int num_levels);
-// Init + the cooperative level loop; returns the grid used.
+// Per-vertex init, then the cooperative level loop (returns the grid used).
+void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);
 // Deviation records, their sort, the lift tables and the nsv pointers.
 void launch_sfxt_index(const SfxtArgs& args, int lift_levels, hipStream_t stream);

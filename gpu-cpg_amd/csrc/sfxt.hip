@@ -391,6 +391,12 @@ int sfxt_lift_levels(int num_levels) {
   return k;
 }
 
+void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream) {
+  hipLaunchKernelGGL(sfxt_init_kernel, dim3(flat_grid(args.n, device_cus())), dim3(256), 0, stream,
+                     args);
+  CPG_HIP(hipGetLastError());
+}
+
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
   int dev = 0, coop = 0;
   CPG_HIP(hipGetDevice(&dev));
@@ -401,8 +407,6 @@ int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
   CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_levels_kernel, kSfxtBlock, 0));
   if (per_cu < 1) throw std::runtime_error("sfxt kernel cannot be resident");
   const int grid = std::max(1, std::min(blocks, cus * per_cu));
-  hipLaunchKernelGGL(sfxt_init_kernel, dim3(flat_grid(args.n, cus)), dim3(256), 0, stream, args);
-  CPG_HIP(hipGetLastError());
   SfxtArgs a = args;
   void* params[] = {&a};
   CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_kernel, dim3(grid), dim3(kSfxtBlock), params, 0,

@@ -76,7 +76,7 @@ typedef struct {
   double walk_count_ms;   /* kernel_timing: count-pass walk kernels, summed              */
   double walk_fill_ms;    /* kernel_timing: fill-pass walk kernels, summed               */
   int64_t walk_launches;  /* kernel_timing: walk kernel launches, both passes            */
-  double sfxt_levels_ms;  /* kernel_timing: SFXT init + level loop (part of sfxt_kernel_ms)  */
+  double sfxt_levels_ms;  /* kernel_timing: SFXT level-loop kernel (part of sfxt_kernel_ms)  */
   int64_t sfxt_blocks;    /* SFXT level-loop grid used                                    */
   int64_t probes;         /* bound-probe count passes run before the first round           */
 } cpg_timing;
