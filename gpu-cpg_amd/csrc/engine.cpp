@@ -115,7 +115,7 @@ struct cpg_handle {
   // PFXT
   DevBuf<float> n_cost;
   DevBuf<int> n_to, n_from, n_parent, n_level;
-  DevBuf<uint32_t> n_key, batch;
+  DevBuf<uint32_t> n_key, batch, ckeys;
   DevBuf<uint8_t> n_flag;
   DevBuf<unsigned> hist_all, hist_pend, hist_part;
   DevBuf<PfxtCtl> pctl;
@@ -284,6 +284,7 @@ struct cpg_handle {
     n_cost.reserve(cap); n_to.reserve(cap); n_from.reserve(cap);
     n_parent.reserve(cap); n_level.reserve(cap);
     n_key.reserve(cap + 16); n_flag.reserve(cap + 16);  // vector loads read whole groups of 8
+    ckeys.reserve(cap + 16);
     batch.reserve(cap);
     hist_all.reserve(2 * kHistBins);
     hist_pend.reserve(kHistBins);
@@ -292,24 +293,26 @@ struct cpg_handle {
     node_cap = cap;
   }
 
-  // Finds a finite first bound without walking the roots' whole chains:
-  // count the roots and their children up to a trial bound c_min + g with a
-  // pruned count pass, and widen g until at least k nonzero paths fall
-  // under it; the bound is then the upper key of the histogram bin where
-  // the running count reaches k (k distinct paths cost no more, so no path
-  // above it can be among the k cheapest).  Returns the bound's key, or
-  // kKeyInf when eight trials fall short or the roots and their children
-  // are fewer than k (the rounds then start unbounded).
   NodeStore node_store() {
     return NodeStore{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, n_key.p, n_flag.p, node_cap};
   }
 
   // Histograms of the stored paths at or below the bound (see launch_hist_nodes).
   void hist_nodes(int mode, bool with_pend, bool accumulate) {
-    launch_hist_nodes(node_store(), pctl.p, mode, with_pend, hist_part.p, hist_all.p, hist_pend.p,
-                      accumulate, stream);
+    launch_hist_nodes(node_store(), ckeys.p, node_cap, pctl.p, mode, with_pend, hist_part.p,
+                      hist_all.p, hist_pend.p, accumulate, stream);
   }
 
+  // Finds a finite first bound without walking the roots' whole chains:
+  // expand the roots with a pruned fill pass that stores every child up to
+  // a trial bound c_min + g, and widen g (dropping the stored children)
+  // until at least k nonzero paths fall under it.  The bound is then the
+  // largest key of the histogram bin where the running count of the stored
+  // paths reaches k (k distinct stored paths cost no more, so no path above
+  // it can be among the k cheapest), and the roots count as expanded.
+  // Returns the bound's key, or kKeyInf when eight trials fall short, the
+  // roots and their children are fewer than k, or a trial overflows the
+  // node store (the rounds then start unbounded from the roots alone).
   uint32_t probe_bound(WalkArgs& wa, int64_t k, const std::vector<uint8_t>& kp,
                        const std::vector<int>& sd, float cmin, uint32_t klo, bool trace) {
     std::vector<float> rc;
@@ -323,47 +326,59 @@ struct cpg_handle {
     gap = std::max({gap, std::fabs(static_cast<double>(cmin)) * 0x1p-8,
                     static_cast<double>(max_abs_distf) * 0x1p-8, 1e-6});
     uint64_t prev_total = 0;
-    wa.strict = 0;  // a probe counts paths <= the trial bound
+    double prev_gap = gap;
+    wa.strict = 0;  // a trial keeps the children at the trial bound itself
     int t = 0;
     for (; t < 8; ++t) {
       const float u0 = static_cast<float>(static_cast<double>(cmin) + gap);
       const uint32_t uk = float_key(u0);
-      launch_probe_setup(pctl.p, uk, window_shift(klo, uk), nb, stream);
-      CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * 2 * kHistBins, stream));
-      launch_walk(wa, false, grid, stream);
+      launch_probe_setup(pctl.p, uk, window_shift(klo, uk), nb, nb, stream);
+      launch_walk(wa, true, grid, stream);
       PfxtCtl pc{};
       d2h(&pc, pctl.p, 1, stream);
       CPG_HIP(hipStreamSynchronize(stream));
       if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
+      if (pc.overflow) break;
       uint64_t roots_le = 0;
       for (float c : rc)
         if (c <= u0 && c != 0.0f) ++roots_le;
-      const uint64_t total = roots_le + pc.child_n;
+      const uint64_t children = pc.node_n - nb;
+      const uint64_t total = roots_le + children;
       if (trace)
         std::fprintf(stderr, "probe %d bound %.9g roots %llu children %llu\n", t,
                      static_cast<double>(u0), static_cast<unsigned long long>(roots_le),
-                     static_cast<unsigned long long>(pc.child_n));
+                     static_cast<unsigned long long>(children));
       if (total >= static_cast<uint64_t>(k)) {
-        // the roots join the children in the histogram, then the crossing
-        hist_nodes(0, false, true);
-        launch_probe_finish(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+        // the stored roots and children give the crossing; the roots are
+        // expanded (every child of theirs up to the trial bound is stored)
+        CPG_HIP(hipMemsetAsync(n_flag.p, 1, nb, stream));
+        hist_nodes(0, false, false);
+        launch_probe_finish(pctl.p, hist_all.p, static_cast<unsigned long long>(k), nb, stream);
         d2h(&pc, pctl.p, 1, stream);
         CPG_HIP(hipStreamSynchronize(stream));
         timing.probes = t + 1;
         return pc.ukey;
       }
-      // a wider bound that counts nothing new means every child of every
-      // root is already counted: k needs deeper paths, stop probing
+      // a wider bound that finds nothing new means every child of every
+      // root is already stored: k needs deeper paths, stop probing
       if (t > 0 && total == prev_total) break;
+      // Counts grow steeply with the bound (children cluster in cost):
+      // model total ~ gap^alpha, alpha fitted to the last two trials (4 at
+      // first), aim 10% past k, and widen by at most 2x so that a
+      // successful trial stores at most the children of twice the needed
+      // width.
+      double alpha = 4.0;
+      if (t > 0 && prev_total > 0 && total > prev_total)
+        alpha = std::min(8.0, std::max(1.0, std::log(static_cast<double>(total) / prev_total) /
+                                                std::log(gap / prev_gap)));
       prev_total = total;
-      // counts grow steeply with the bound (children cluster in cost), so
-      // widen by at most 2x: the last, successful trial then costs at most
-      // a pruned pass at twice the needed width
-      const double f = std::sqrt(static_cast<double>(k) / static_cast<double>(std::max<uint64_t>(total, 1))) * 1.1;
-      gap *= std::min(2.0, std::max(1.25, f));
+      prev_gap = gap;
+      const double f = std::pow(static_cast<double>(k) / static_cast<double>(std::max<uint64_t>(total, 1)),
+                                1.0 / alpha) * 1.1;
+      gap *= std::min(2.0, std::max(1.1, f));
     }
     timing.probes = std::min(t + 1, 8);
-    launch_probe_setup(pctl.p, kKeyInf, window_shift(klo, kKeyInf), 0, stream);
+    launch_probe_setup(pctl.p, kKeyInf, window_shift(klo, kKeyInf), 0, nb, stream);
     CPG_HIP(hipMemsetAsync(hist_all.p, 0, sizeof(unsigned) * 2 * kHistBins, stream));
     return kKeyInf;
   }
@@ -453,7 +468,7 @@ struct cpg_handle {
       launch_roots(srcs.p, keep.p, nsrc, dist.p, distf.p, ns, pctl.p, stream);
       WalkArgs wa{};
       wa.vrec = vrec.p; wa.devinfo = devinfo.p; wa.devs = devs.p; wa.batch = batch.p;
-      wa.nodes = ns; wa.hist = hist_all.p; wa.ctl = pctl.p;
+      wa.nodes = ns; wa.ckeys = ckeys.p; wa.ckeys_cap = cap; wa.ctl = pctl.p;
       wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = g.max_abs_wgt;
       wa.collect_stats = o.collect_stats;
       wa.step_limit = static_cast<int>(std::min<int64_t>(
@@ -465,41 +480,53 @@ struct cpg_handle {
       if (probe) ukey_host = probe_bound(wa, k, kp, sd, minc, init.klo, trace);
       // A bound set from stored paths lets a fill pass drop children at the
       // bound (k stored paths already cost no more).  A bound set from
-      // counted children that are not stored yet (the probe, a count pass)
-      // needs an inclusive fill, or the paths at the bound would be lost.
-      bool bound_from_store = false;
+      // counted children that are not stored yet (a count pass) needs an
+      // inclusive fill, or the paths at the bound would be lost.  The probe
+      // sets its bound from stored paths.
+      const bool probed = ukey_host != kKeyInf;
+      bool bound_from_store = probed;
       bool overflow = false;
-      for (int r = 0;; ++r) {
+      int r = 0;
+      while (true) {
         if (r > 100000) throw std::runtime_error("PFXT round limit exceeded");
         if (now_ms() - tq0 > 120000.0) throw std::runtime_error("PFXT query exceeded 120 s");
-        hist_nodes(0, true, false);
-        launch_select_batch(pctl.p, hist_pend.p, r == 0 ? ~0ull : batch_target, stream);
-        launch_make_batch(ns, pctl.p, batch.p, grid, stream);
-        if (kernel_timing) {
-          ev_count.create();
-          ev_fill.create();
-          CPG_HIP(hipEventRecord(ev_count.a, stream));
-        }
+        // Once the bound is finite and set from stored paths, every round is
+        // the same fill-only sequence and every kernel does nothing after
+        // the done flag is set: launch a group of rounds per host sync.
         const bool count_pass = two_pass || ukey_host == kKeyInf;
-        if (count_pass) {
-          wa.strict = ukey_host == kKeyInf || bound_from_store ? 1 : 0;
-          launch_walk(wa, false, grid, stream);
-          if (kernel_timing) CPG_HIP(hipEventRecord(ev_count.b, stream));
-          launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
-          if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.a, stream));
-          wa.strict = 0;
-          launch_walk(wa, true, grid, stream);
-          if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
-        } else {
-          wa.strict = bound_from_store ? 1 : 0;
+        const int group = (!kernel_timing && !count_pass && bound_from_store) ? 4 : 1;
+        for (int gi = 0; gi < group; ++gi, ++r) {
+          hist_nodes(0, true, false);
+          // the first round expands every root, unless the probe already did
+          const bool all_roots = r == 0 && !probed;
+          launch_select_batch(pctl.p, hist_pend.p, all_roots ? ~0ull : batch_target, stream);
+          launch_make_batch(ns, pctl.p, batch.p, grid, stream);
           if (kernel_timing) {
-            CPG_HIP(hipEventRecord(ev_count.b, stream));
-            CPG_HIP(hipEventRecord(ev_fill.a, stream));
+            ev_count.create();
+            ev_fill.create();
+            CPG_HIP(hipEventRecord(ev_count.a, stream));
           }
-          launch_walk(wa, true, grid, stream);
-          if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
-          hist_nodes(1, false, true);
-          launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+          if (count_pass) {
+            wa.strict = ukey_host == kKeyInf || bound_from_store ? 1 : 0;
+            launch_walk(wa, false, grid, stream);
+            hist_nodes(2, false, true);
+            if (kernel_timing) CPG_HIP(hipEventRecord(ev_count.b, stream));
+            launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+            if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.a, stream));
+            wa.strict = 0;
+            launch_walk(wa, true, grid, stream);
+            if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
+          } else {
+            wa.strict = bound_from_store ? 1 : 0;
+            if (kernel_timing) {
+              CPG_HIP(hipEventRecord(ev_count.b, stream));
+              CPG_HIP(hipEventRecord(ev_fill.a, stream));
+            }
+            launch_walk(wa, true, grid, stream);
+            if (kernel_timing) CPG_HIP(hipEventRecord(ev_fill.b, stream));
+            hist_nodes(1, false, true);
+            launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
+          }
         }
         d2h(&c, pctl.p, 1, stream);
         CPG_HIP(hipStreamSynchronize(stream));
@@ -512,11 +539,11 @@ struct cpg_handle {
             std::fprintf(stderr,
                          "round %d %s batch %llu expanded %llu nodes %llu U %.9g "
                          "count %.3f ms fill %.3f ms\n",
-                         r, count_pass ? "count+fill" : "fill", c.batch_n, c.expanded, c.node_n,
+                         r - 1, count_pass ? "count+fill" : "fill", c.batch_n, c.expanded, c.node_n,
                          static_cast<double>(key_float(c.ukey)), cm, fm);
         }
         ukey_host = c.ukey;
-        bound_from_store = true;  // every path counted this round is stored now
+        bound_from_store = true;  // every path counted so far is stored now
         if (c.err) throw std::runtime_error("PFXT walk exceeded its step bound");
         if (c.overflow) { overflow = true; break; }
         if (c.done || c.batch_n == 0) break;  // no unexpanded path below the bound

@@ -17,8 +17,9 @@ struct SfxtCtl {
   unsigned levelized;           // vertices peeled
   unsigned max_abs_distf_bits;  // max |distf| as float bits
   int nsv_err;                  // 1: nsv descent found no answer, 2: long list without devs_tmp
-  unsigned nbig;                // deviation lists longer than one thread sorts
   unsigned pad;
+  unsigned long long nbig;      // deviation lists one wave sorts (front of big[])
+  unsigned long long nbig_l;    // deviation lists one block sorts (back of big[])
 };
 
 struct SfxtArgs {
@@ -96,7 +97,8 @@ struct WalkArgs {
   const int4* devs;
   const uint32_t* batch;
   NodeStore nodes;
-  unsigned* hist;         // count pass histogram (2 x kHistBins: counts, per-bin max key)
+  uint32_t* ckeys;        // count pass: children's keys (ckeys_cap slots, padded by 16)
+  uint64_t ckeys_cap;
   PfxtCtl* ctl;
   float max_abs_distf;
   float max_abs_wgt;
@@ -108,16 +110,18 @@ struct WalkArgs {
 void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dist,
                   const float* distf, NodeStore nodes, PfxtCtl* ctl, hipStream_t s);
 void launch_probe_setup(PfxtCtl* ctl, uint32_t ukey, uint32_t shift, unsigned long long nb,
-                        hipStream_t s);
+                        unsigned long long nroots, hipStream_t s);
 void launch_iota(uint32_t* p, uint64_t n, int grid, hipStream_t s);
-void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s);
-// Histograms of stored paths (mode 0: all, 1: this round's children) into
+void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
+                         unsigned long long nroots, hipStream_t s);
+// Histograms of stored paths (mode 0: all, 1: this round's children, 2: the
+// keys of the last count pass) into
 // hist_all (2 x kHistBins: counts of nonzero keys <= U, then each bin's
 // largest key) and, with_pend, hist_pend (unexpanded paths with key < U).
 // part holds 3 x kHistParts x kHistBins per-block partials.
-void launch_hist_nodes(const NodeStore& nodes, PfxtCtl* ctl, int mode, bool with_pend,
-                       unsigned* part, unsigned* hist_all, unsigned* hist_pend, bool accumulate,
-                       hipStream_t s);
+void launch_hist_nodes(const NodeStore& nodes, const uint32_t* ckeys, uint64_t ckeys_cap,
+                       PfxtCtl* ctl, int mode, bool with_pend, unsigned* part, unsigned* hist_all,
+                       unsigned* hist_pend, bool accumulate, hipStream_t s);
 void launch_select_batch(PfxtCtl* ctl, unsigned* hist_pend, unsigned long long batch_target,
                          hipStream_t s);
 void launch_make_batch(const NodeStore& nodes, PfxtCtl* ctl, uint32_t* batch, int grid,

@@ -114,12 +114,14 @@ __global__ void __launch_bounds__(kBlock) roots_kernel(const int* srcs, const ui
 // written out as one partial per block: the nonzero keys <= U with the
 // largest key of each bin, and (with_pend) the unexpanded paths with key
 // < U (those still to be expanded).  mode 0 covers every stored path,
-// mode 1 this round's children only.  Each thread reads four consecutive
-// keys and flags with one vector load each (the store's arrays are padded
-// to a multiple of four slots).
-__global__ void __launch_bounds__(kBlock) hist_nodes_kernel(NodeStore nodes, PfxtCtl* ctl, int mode,
-                                                            int with_pend, unsigned* part_all,
-                                                            unsigned* part_pend,
+// mode 1 this round's children only, mode 2 the keys a count pass wrote
+// (ckeys, no flags).  Each thread reads four consecutive keys and flags
+// with one vector load each (the arrays are padded to a multiple of four
+// slots).
+__global__ void __launch_bounds__(kBlock) hist_nodes_kernel(NodeStore nodes, const uint32_t* ckeys,
+                                                            uint64_t ckeys_cap, PfxtCtl* ctl,
+                                                            int mode, int with_pend,
+                                                            unsigned* part_all, unsigned* part_pend,
                                                             unsigned* part_max) {
   __shared__ unsigned h_all[kHistBins];
   __shared__ unsigned h_pend[kHistBins];
@@ -130,16 +132,22 @@ __global__ void __launch_bounds__(kBlock) hist_nodes_kernel(NodeStore nodes, Pfx
     h_max[j] = 0;
   }
   __syncthreads();
-  const uint64_t nn = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
-  const uint64_t lo = mode == 0 ? 0 : ctl->fill_node_base;
+  uint64_t nn, lo = 0;
+  if (mode == 2) {
+    nn = ctl->child_n < ckeys_cap ? ctl->child_n : ckeys_cap;
+  } else {
+    nn = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
+    if (mode == 1) lo = ctl->fill_node_base;
+  }
+  if (ctl->done) nn = lo;  // nothing left: write empty partials
   const uint32_t uk = ctl->ukey;
   const uint32_t klo = ctl->klo, shift = ctl->shift;
-  const uint4* key4 = reinterpret_cast<const uint4*>(nodes.key);
+  const uint4* key4 = reinterpret_cast<const uint4*>(mode == 2 ? ckeys : nodes.key);
   const uint32_t* flag4 = reinterpret_cast<const uint32_t*>(nodes.flag);
   for (uint64_t g = lo / 4 + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        g < (nn + 3) / 4; g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint4 kv = key4[g];
-    const uint32_t fv = flag4[g];
+    const uint32_t fv = with_pend ? flag4[g] : 0u;
     const uint32_t ks[4] = {kv.x, kv.y, kv.z, kv.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -266,18 +274,21 @@ __global__ void __launch_bounds__(kBlock) make_batch_kernel(NodeStore nodes, Pfx
 constexpr int kStage = 256;  // per-wave LDS slots for children awaiting a flush
 
 // Walks the suffix chains of the batch's parents and finds every child
-// with cost < U (cost <= U for a bound probe, a.strict == 0).  Count pass (kFill = false): histogram the children's
-// keys.  Fill pass: append them to the node store, unexpanded.  Children
-// are staged per wave in LDS and written with one counter atomic per flush.
+// with cost < U (cost <= U when a.strict == 0).  Count pass (kFill =
+// false): write the children's keys to ckeys and count them in child_n
+// (hist_nodes_kernel mode 2 histograms them).  Fill pass: append them to
+// the node store, unexpanded.  Children are staged per wave in LDS and
+// written with one counter atomic per flush.
 template <bool kFill>
 __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
   constexpr int kWaves = kBlock / 64;
-  constexpr int kS = kFill ? kStage : 1;
+  constexpr int kS = kStage;
+  constexpr int kSF = kFill ? kStage : 1;  // node fields are staged by the fill pass only
   __shared__ float s_cost[kWaves][kS];
-  __shared__ int s_to[kWaves][kS];
-  __shared__ int s_from[kWaves][kS];
-  __shared__ int s_parent[kWaves][kS];
-  __shared__ int s_level[kWaves][kS];
+  __shared__ int s_to[kWaves][kSF];
+  __shared__ int s_from[kWaves][kSF];
+  __shared__ int s_parent[kWaves][kSF];
+  __shared__ int s_level[kWaves][kSF];
   PfxtCtl* ctl = a.ctl;
   if (ctl->done) return;
   const int wave = threadIdx.x >> 6;
@@ -286,20 +297,28 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
   const uint32_t ukey = ctl->ukey;
   const bool bounded = ukey != kKeyInf;
   const float U = key_float(ukey);
-  const uint32_t klo = ctl->klo, shift = ctl->shift;
   const uint64_t cap = a.nodes.cap;
   const float mag = 2.0f * a.max_abs_distf + a.max_abs_wgt + 1.0f;
-  unsigned long long hops = 0, devv = 0, counted = 0;
+  unsigned long long hops = 0, devv = 0;
   int staged = 0;  // wave-uniform
 
-  // Writes the wave's staged children: one atomic on node_n, then each
-  // lane stores every 64th child.  All lanes of the wave call it together.
+  // Writes the wave's staged children: one atomic on node_n (fill) or
+  // child_n (count: keys only, into ckeys), then each lane stores every
+  // 64th child.  All lanes of the wave call it together.
   auto flush = [&]() {
-    if (!kFill || staged == 0) return;
+    if (staged == 0) return;
     __builtin_amdgcn_wave_barrier();
     unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(&ctl->node_n, static_cast<unsigned long long>(staged));
+    if (lane == 0)
+      base = atomicAdd(kFill ? &ctl->node_n : &ctl->child_n, static_cast<unsigned long long>(staged));
     base = __shfl(base, 0);
+    if (!kFill) {
+      for (int q = lane; q < staged; q += 64)
+        if (base + q < a.ckeys_cap) a.ckeys[base + q] = float_key(s_cost[wave][q]);
+      __builtin_amdgcn_wave_barrier();
+      staged = 0;
+      return;
+    }
     for (int q = lane; q < staged; q += 64) {
       const uint64_t slot = base + q;
       const float cost = s_cost[wave][q];
@@ -382,37 +401,25 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
           mode = v == -1 ? 2 : 0;
         }
       }
-      if (kFill) {
-        const uint64_t m = __ballot(emit);
-        if (m) {
-          const int add = static_cast<int>(__popcll(m));
-          if (staged + add > kS) flush();
-          if (emit) {
-            const int q = staged + static_cast<int>(__popcll(m & lanemask_lt()));
-            s_cost[wave][q] = cc;
+      const uint64_t m = __ballot(emit);
+      if (m) {
+        const int add = static_cast<int>(__popcll(m));
+        if (staged + add > kS) flush();
+        if (emit) {
+          const int q = staged + static_cast<int>(__popcll(m & lanemask_lt()));
+          s_cost[wave][q] = cc;
+          if (kFill) {
             s_to[wave][q] = cw;
             s_from[wave][q] = from;
             s_parent[wave][q] = p;
             s_level[wave][q] = lvl + 1;
           }
-          staged += add;
         }
-      } else if (emit) {
-        ++counted;
-        const uint32_t k = float_key(cc);
-        if (!key_is_zero(k)) {
-          const uint32_t b = key_bin(k, klo, shift);
-          atomicAdd(&a.hist[b], 1u);
-          atomicMax(&a.hist[kHistBins + b], k);
-        }
+        staged += add;
       }
     }
   }
   flush();
-  if (!kFill) {
-    for (int o = 32; o > 0; o >>= 1) counted += __shfl_xor(counted, o);
-    if (lane == 0 && counted) atomicAdd(&ctl->child_n, counted);
-  }
   if (a.collect_stats) {
     for (int o = 32; o > 0; o >>= 1) {
       hops += __shfl_xor(hops, o);
@@ -446,14 +453,18 @@ __global__ void __launch_bounds__(kSelBlock) select_bound_kernel(PfxtCtl* ctl, u
 }
 
 // Bound probe: the walk's inputs for one trial bound ukey over the roots
-// (node slots 0..nb-1, listed in batch by iota_kernel).
+// (node slots 0..nroots-1, listed in batch by iota_kernel); the node store
+// is cut back to the roots, dropping an earlier trial's children.
 __global__ void probe_setup_kernel(PfxtCtl* ctl, uint32_t ukey, uint32_t shift,
-                                   unsigned long long nb) {
+                                   unsigned long long nb, unsigned long long nroots) {
   ctl->ukey = ukey;
   ctl->shift = shift;
   ctl->batch_n = nb;
   ctl->child_n = 0;
   ctl->done = 0;
+  ctl->node_n = nroots;
+  ctl->fill_node_base = nroots;
+  ctl->overflow = 0;
 }
 
 __global__ void iota_kernel(uint32_t* p, uint64_t n) {
@@ -463,10 +474,11 @@ __global__ void iota_kernel(uint32_t* p, uint64_t n) {
 }
 
 // Bound probe succeeded (at least k nonzero paths with cost <= the trial
-// bound were counted): the bound becomes the largest key of the bin where
-// the running count reaches k, and the probe's batch is dropped.
+// bound are stored): the bound becomes the largest key of the bin where
+// the running count reaches k, and the nroots roots count as expanded.
 __global__ void __launch_bounds__(kSelBlock) probe_finish_kernel(PfxtCtl* ctl, unsigned* hist_all,
-                                                                 unsigned long long k) {
+                                                                 unsigned long long k,
+                                                                 unsigned long long nroots) {
   unsigned long long total = 0;
   const uint32_t b = find_crossing(hist_all, k, &total);
   if (threadIdx.x == 0) {
@@ -479,6 +491,7 @@ __global__ void __launch_bounds__(kSelBlock) probe_finish_kernel(PfxtCtl* ctl, u
     ctl->shift = window_shift(ctl->klo, uk);
     ctl->batch_n = 0;
     ctl->child_n = 0;
+    ctl->expanded += nroots;
   }
   __syncthreads();
   clear_hist(hist_all);
@@ -496,8 +509,8 @@ void launch_roots(const int* srcs, const uint8_t* keep, int nsrc, const int* dis
 }
 
 void launch_probe_setup(PfxtCtl* ctl, uint32_t ukey, uint32_t shift, unsigned long long nb,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(probe_setup_kernel, dim3(1), dim3(1), 0, s, ctl, ukey, shift, nb);
+                        unsigned long long nroots, hipStream_t s) {
+  hipLaunchKernelGGL(probe_setup_kernel, dim3(1), dim3(1), 0, s, ctl, ukey, shift, nb, nroots);
   CPG_HIP(hipGetLastError());
 }
 
@@ -507,20 +520,21 @@ void launch_iota(uint32_t* p, uint64_t n, int grid, hipStream_t s) {
   CPG_HIP(hipGetLastError());
 }
 
-void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s) {
-  hipLaunchKernelGGL(probe_finish_kernel, dim3(1), dim3(kSelBlock), 0, s, ctl, hist_all, k);
+void launch_probe_finish(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
+                         unsigned long long nroots, hipStream_t s) {
+  hipLaunchKernelGGL(probe_finish_kernel, dim3(1), dim3(kSelBlock), 0, s, ctl, hist_all, k, nroots);
   CPG_HIP(hipGetLastError());
 }
 
-void launch_hist_nodes(const NodeStore& nodes, PfxtCtl* ctl, int mode, bool with_pend,
-                       unsigned* part, unsigned* hist_all, unsigned* hist_pend, bool accumulate,
-                       hipStream_t s) {
+void launch_hist_nodes(const NodeStore& nodes, const uint32_t* ckeys, uint64_t ckeys_cap,
+                       PfxtCtl* ctl, int mode, bool with_pend, unsigned* part, unsigned* hist_all,
+                       unsigned* hist_pend, bool accumulate, hipStream_t s) {
   const size_t plane = static_cast<size_t>(kHistParts) * kHistBins;
   unsigned* part_all = part;
   unsigned* part_pend = part + plane;
   unsigned* part_max = part + 2 * plane;
-  hipLaunchKernelGGL(hist_nodes_kernel, dim3(kHistParts), dim3(kBlock), 0, s, nodes, ctl, mode,
-                     with_pend ? 1 : 0, part_all, part_pend, part_max);
+  hipLaunchKernelGGL(hist_nodes_kernel, dim3(kHistParts), dim3(kBlock), 0, s, nodes, ckeys,
+                     ckeys_cap, ctl, mode, with_pend ? 1 : 0, part_all, part_pend, part_max);
   CPG_HIP(hipGetLastError());
   const int rg = kHistBins / 64;
   const int acc = accumulate ? 1 : 0;

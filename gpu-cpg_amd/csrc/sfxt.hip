@@ -40,8 +40,10 @@ namespace {
 constexpr int kSfxtBlock = 256;
 constexpr int kLdsQueue = 2048;
 constexpr unsigned kSpinLimit = 1u << 22;
-constexpr int kThreadSort = 32;    // deviation lists up to this length: one thread sorts
+constexpr int kThreadSort = 16;    // deviation lists up to this length: one thread sorts
+constexpr int kWaveSort = 64;      // up to this length: one wave ranks them in registers
 constexpr int kLdsSort = kSfxtLdsSort;  // up to this length: one block sorts in LDS
+constexpr int kFaninUnroll = 8;   // fanin records (and their atomics) in flight per step
 constexpr int kSortBlock = 256;
 
 __device__ inline unsigned atomic_load_rlx(unsigned* p) {
@@ -138,7 +140,8 @@ __global__ void __launch_bounds__(256) sfxt_init_kernel(SfxtArgs a) {
 // relax and successor rules above, read with one load instead of a gather
 // over x's fanouts.  Per vertex the level loop makes four dependent memory
 // round trips (queue entry; best and fanin range; fanin records; the
-// atomics), with all loads of a round issued together.
+// atomics), with all loads of a round issued together (kFaninUnroll fanins
+// per step).
 __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
   __shared__ int s_q[kLdsQueue];
   __shared__ int s_qn;
@@ -185,24 +188,24 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
       a.level[u] = d;
       a.distf[u] = df;
       local_max = fmaxf(local_max, fabsf(df));
-      for (int j = b; j < e; j += 4) {
-        // four fanin records; past the end, repeat the last one with a
-        // no-op min and a zero decrement, so every load is unconditional
-        int2 r[4];
+      for (int j = b; j < e; j += kFaninUnroll) {
+        // kFaninUnroll fanin records; past the end the last one is loaded
+        // again (so every load is unconditional) and its atomics are skipped
+        int2 r[kFaninUnroll];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] = a.iedge[min(j + q, e - 1)];
-        unsigned long long cand[4];
-        int old[4];
+        for (int q = 0; q < kFaninUnroll; ++q) r[q] = a.iedge[min(j + q, e - 1)];
+        int old[kFaninUnroll];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const bool ok = j + q < e;
-          const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
-          cand[q] = ok ? pack_cand(du + wi, u) : ~0ull;
-          atomicMin(&a.best[r[q].x], cand[q]);
-          old[q] = atomicSub(&a.deps[r[q].x], ok ? 1 : 0);
+        for (int q = 0; q < kFaninUnroll; ++q) {
+          old[q] = 0;
+          if (j + q < e) {
+            const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
+            atomicMin(&a.best[r[q].x], pack_cand(du + wi, u));
+            old[q] = atomicSub(&a.deps[r[q].x], 1);
+          }
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < kFaninUnroll; ++q)
           if (j + q < e && old[q] == 1) push(r[q].x);
       }
     }
@@ -226,73 +229,121 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
 // Deviation records of every settled vertex in fanout order, their smallest
 // delta, and the vertex record without its nsv pointer.  Lists of up to
 // kThreadSort records are sorted here (stable insertion sort by delta);
-// longer ones are queued for sfxt_sort_big_kernel.
+// longer ones are queued for sfxt_sort_mid_kernel (up to kWaveSort, from
+// the front of big[]) or sfxt_sort_big_kernel (from the back).  The loop
+// is wave-uniform so the queue reservations are wave-aggregated.
 __global__ void __launch_bounds__(256) sfxt_devs_kernel(SfxtArgs a) {
-  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < a.n; u += gridDim.x * blockDim.x) {
-    const int b = a.oadjp[u], e = a.oadjp[u + 1];
-    if (a.level[u] < 0) {
-      a.vrec[u] = default_vrec();
-      a.devinfo[u] = make_int2(b, 0);
-      continue;
-    }
-    const int s = a.succ[u];
-    const float df = a.distf[u];
+  for (int base = blockIdx.x * blockDim.x; base < a.n; base += gridDim.x * blockDim.x) {
+    const int u = base + threadIdx.x;
     int cnt = 0;
-    float mind = __builtin_huge_valf();
-    for (int j = b; j < e; ++j) {
-      const int w = a.oadj[j];
-      if (w == s) continue;
-      if (a.dist[w] == kIntMax) continue;
-      const float fw = a.distf[w];
-      const float we = a.owgt[j];
-      const float delta = static_cast<float>(static_cast<double>(fw) + static_cast<double>(we) -
-                                             static_cast<double>(df));
-      a.devs[b + cnt] = make_int4(w, __float_as_int(we), __float_as_int(fw), __float_as_int(delta));
-      mind = fminf(mind, delta);
-      ++cnt;
-    }
-    if (cnt <= kThreadSort) {
-      for (int i = 1; i < cnt; ++i) {
-        const int4 rec = a.devs[b + i];
-        const float key = __int_as_float(rec.w);
-        int k = i;
-        while (k > 0) {
-          const int4 prev = a.devs[b + k - 1];
-          if (__int_as_float(prev.w) <= key) break;
-          a.devs[b + k] = prev;
-          --k;
+    bool sorted = true;
+    if (u < a.n) {
+      const int b = a.oadjp[u], e = a.oadjp[u + 1];
+      if (a.level[u] < 0) {
+        a.vrec[u] = default_vrec();
+        a.devinfo[u] = make_int2(b, 0);
+      } else {
+        const int s = a.succ[u];
+        const float df = a.distf[u];
+        float mind = __builtin_huge_valf();
+        for (int j = b; j < e; ++j) {
+          const int w = a.oadj[j];
+          if (w == s) continue;
+          if (a.dist[w] == kIntMax) continue;
+          const float fw = a.distf[w];
+          const float we = a.owgt[j];
+          const float delta = static_cast<float>(static_cast<double>(fw) +
+                                                 static_cast<double>(we) -
+                                                 static_cast<double>(df));
+          a.devs[b + cnt] =
+              make_int4(w, __float_as_int(we), __float_as_int(fw), __float_as_int(delta));
+          mind = fminf(mind, delta);
+          ++cnt;
         }
-        if (k != i) a.devs[b + k] = rec;
+        if (cnt <= kThreadSort) {
+          for (int i = 1; i < cnt; ++i) {
+            const int4 rec = a.devs[b + i];
+            const float key = __int_as_float(rec.w);
+            int k = i;
+            while (k > 0) {
+              const int4 prev = a.devs[b + k - 1];
+              if (__int_as_float(prev.w) <= key) break;
+              a.devs[b + k] = prev;
+              --k;
+            }
+            if (k != i) a.devs[b + k] = rec;
+          }
+        } else {
+          sorted = false;
+        }
+        a.vrec[u] = make_int4(__float_as_int(mind), -1, s, __float_as_int(df));
+        a.devinfo[u] = make_int2(b, cnt);
       }
-    } else {
-      a.big[atomicAdd(&a.ctl->nbig, 1u)] = u;
     }
-    a.vrec[u] = make_int4(__float_as_int(mind), -1, s, __float_as_int(df));
-    a.devinfo[u] = make_int2(b, cnt);
+    const bool mid = !sorted && cnt <= kWaveSort;
+    const bool large = !sorted && cnt > kWaveSort;
+    const uint64_t sm = wave_reserve(reinterpret_cast<unsigned long long*>(&a.ctl->nbig), mid);
+    const uint64_t sl = wave_reserve(reinterpret_cast<unsigned long long*>(&a.ctl->nbig_l), large);
+    if (mid) a.big[sm] = u;
+    if (large) a.big[a.n - 1 - static_cast<int>(sl)] = u;
   }
 }
 
-// Stable sort by delta of the long deviation lists, one block per list.
-// A record's rank is the number of records with a smaller delta plus the
-// number of earlier records with an equal delta, which reproduces a stable
-// insertion sort.  Lists of up to kLdsSort records are ranked in LDS; longer
-// ones (only when the graph has such fanouts) rank from global memory and
-// permute through devs_tmp, the same slot range of a second m-record buffer.
+// Total order used to rank deviation records: float order with -0 == +0
+// (ties keep fanout order, as the insertion sort does); NaN ranks last.
+__device__ inline uint32_t delta_rank_key(int bits) {
+  const float x = __int_as_float(bits);
+  return x == 0.0f ? kKeyPosZero : float_key(x);
+}
+
+// Stable sort by delta of the lists of kThreadSort+1 .. kWaveSort records,
+// one wave per list: lane i holds record i and its rank is the number of
+// records that order before it (shuffled keys), so the records land in
+// place with one store each.
+__global__ void __launch_bounds__(256) sfxt_sort_mid_kernel(SfxtArgs a) {
+  const unsigned nmid = static_cast<unsigned>(a.ctl->nbig);
+  const int lane = __lane_id();
+  const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (unsigned t = wave; t < nmid; t += nwaves) {
+    const int u = a.big[t];
+    const int2 di = a.devinfo[u];
+    const int b = di.x, cnt = di.y;
+    int4 rec = make_int4(0, 0, 0, 0);
+    uint32_t key = 0xffffffffu;
+    if (lane < cnt) {
+      rec = a.devs[b + lane];
+      key = delta_rank_key(rec.w);
+    }
+    int rank = 0;
+    for (int j = 0; j < cnt; ++j) {
+      const uint32_t kj = __shfl(key, j);
+      rank += (kj < key || (kj == key && j < lane)) ? 1 : 0;
+    }
+    if (lane < cnt) a.devs[b + rank] = rec;
+  }
+}
+
+// Stable sort by delta of the lists longer than kWaveSort, one block per
+// list (ranks as in sfxt_sort_mid_kernel).  Lists of up to kLdsSort records
+// are ranked in LDS; longer ones (only when the graph has such fanouts)
+// rank from global memory and permute through devs_tmp, the same slot
+// range of a second m-record buffer.
 __global__ void __launch_bounds__(kSortBlock) sfxt_sort_big_kernel(SfxtArgs a) {
   __shared__ int4 s_rec[kLdsSort];
-  const unsigned nbig = a.ctl->nbig;
-  for (unsigned t = blockIdx.x; t < nbig; t += gridDim.x) {
-    const int u = a.big[t];
+  const unsigned nl = static_cast<unsigned>(a.ctl->nbig_l);
+  for (unsigned t = blockIdx.x; t < nl; t += gridDim.x) {
+    const int u = a.big[a.n - 1 - static_cast<int>(t)];
     const int2 di = a.devinfo[u];
     const int b = di.x, cnt = di.y;
     if (cnt <= kLdsSort) {
       for (int i = threadIdx.x; i < cnt; i += blockDim.x) s_rec[i] = a.devs[b + i];
       __syncthreads();
       for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-        const float key = __int_as_float(s_rec[i].w);
+        const uint32_t key = delta_rank_key(s_rec[i].w);
         int rank = 0;
         for (int j = 0; j < cnt; ++j) {
-          const float kj = __int_as_float(s_rec[j].w);
+          const uint32_t kj = delta_rank_key(s_rec[j].w);
           rank += (kj < key || (kj == key && j < i)) ? 1 : 0;
         }
         a.devs[b + rank] = s_rec[i];
@@ -301,10 +352,10 @@ __global__ void __launch_bounds__(kSortBlock) sfxt_sort_big_kernel(SfxtArgs a) {
     } else if (a.devs_tmp) {
       for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
         const int4 rec = a.devs[b + i];
-        const float key = __int_as_float(rec.w);
+        const uint32_t key = delta_rank_key(rec.w);
         int rank = 0;
         for (int j = 0; j < cnt; ++j) {
-          const float kj = __int_as_float(a.devs[b + j].w);
+          const uint32_t kj = delta_rank_key(a.devs[b + j].w);
           rank += (kj < key || (kj == key && j < i)) ? 1 : 0;
         }
         a.devs_tmp[b + rank] = rec;
@@ -418,6 +469,8 @@ void launch_sfxt_index(const SfxtArgs& args, int lift_levels, hipStream_t stream
   const int cus = device_cus();
   const int g = flat_grid(args.n, cus);
   hipLaunchKernelGGL(sfxt_devs_kernel, dim3(g), dim3(256), 0, stream, args);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sfxt_sort_mid_kernel, dim3(cus * 8), dim3(256), 0, stream, args);
   CPG_HIP(hipGetLastError());
   hipLaunchKernelGGL(sfxt_sort_big_kernel, dim3(cus * 2), dim3(kSortBlock), 0, stream, args);
   CPG_HIP(hipGetLastError());

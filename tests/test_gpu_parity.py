@@ -271,3 +271,40 @@ def test_c3_unit_weights_sorted_multiset_prefixes():
     got = g.get_slacks(1_000_000)
     say(f"C3 unit timings {g.timings()}")
     assert_multiset_prefixes(want, got, "C3 unit", (1000, 10_000, 50_000, 100_000, 1_000_000))
+
+
+def medium_fanout_graph():
+    """3000 vertices in 20 levels, out-degree up to ~90 (distinct targets
+    1-3 levels ahead), weights mixing exact ties (0.5, 1.0) and uniform
+    draws: deviation lists of every length class (thread, wave, block
+    sort)."""
+    rng = np.random.default_rng(11)
+    levels, per = 20, 150
+    n = levels * per
+    u, v, w = [], [], []
+    for lv in range(levels - 1):
+        for i in range(per):
+            x = lv * per + i
+            deg = int(rng.integers(1, 90))
+            tl = rng.integers(lv + 1, min(levels, lv + 4), size=deg)
+            tg = tl * per + rng.integers(0, per, size=deg)
+            for t in np.unique(tg):
+                u.append(x); v.append(int(t)); w.append(float(rng.choice([0.5, 1.0, rng.uniform(0.01, 1)])))
+    return n, np.array(u, np.int32), np.array(v, np.int32), np.array(w, np.float32)
+
+
+def test_medium_fanout_deviation_sorts():
+    n, u, v, w = medium_fanout_graph()
+    say("case medium fanout (<= 90)")
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, level, L = og.sfxt()
+    g = engine_for(n, u, v, w)
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    for k in (10, 1000, 20000):
+        want, _ = og.topk(dist, succ, k)
+        g.reset()
+        g.report_paths(k)
+        assert_bits_equal(want, g.get_slacks(k), f"medium fanout K={k}")
