@@ -40,7 +40,8 @@ class Timing(ctypes.Structure):
                 ("sfxt_kernel_ms", ctypes.c_double), ("walk_count_ms", ctypes.c_double),
                 ("walk_fill_ms", ctypes.c_double), ("walk_launches", ctypes.c_int64),
                 ("sfxt_levels_ms", ctypes.c_double), ("sfxt_blocks", ctypes.c_int64),
-                ("probes", ctypes.c_int64)]
+                ("probes", ctypes.c_int64), ("bvss_ms", ctypes.c_double),
+                ("bvss_tiles", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -78,6 +79,12 @@ ABI = {
     "cpg_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "cpg_sfxt": (ctypes.c_int, [_H]),
     "cpg_get_sfxt": (ctypes.c_int, [_H, _PI, _PI, _PI]),
+    "cpg_bvss_build": (ctypes.c_int, [_H]),
+    "cpg_bvss_info": (ctypes.c_int, [_H, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                     ctypes.POINTER(_I64)]),
+    "cpg_bvss_row_sources": (_I64, [_H, ctypes.c_int, _PI, _I64]),
+    "cpg_bvss_discover": (_I64, [_H, ctypes.c_int, _PI, _PI, ctypes.c_int, ctypes.c_int,
+                                 _PI, _PI, _PI, _I64]),
     "cpg_cache_enable": (ctypes.c_int, [_H, ctypes.c_int]),
     "cpg_cache_clear": (ctypes.c_int, [_H]),
     "cpg_cache_hits": (ctypes.c_int, [_H]),
@@ -283,6 +290,50 @@ class CpGen:
 
     def run_sfxt(self):
         _c(self, self.L.cpg_sfxt(self.h))
+
+    # -- BVSS deviation discovery (gpucpg/tc_pfxt_bvss.cuh)
+    def bvss_build(self):
+        _c(self, self.L.cpg_bvss_build(self.h))
+
+    def bvss_info(self):
+        a, b, c = _I64(), _I64(), _I64()
+        _c(self, self.L.cpg_bvss_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"intervals": a.value, "entries": b.value, "set_bits": c.value}
+
+    def bvss_row_sources(self, row):
+        cap = 64
+        while True:
+            out = np.zeros(cap, np.int32)
+            n = self.L.cpg_bvss_row_sources(self.h, int(row), _iptr(out), cap)
+            if n >= 0:
+                return out[:n]
+            if n <= -1000:
+                cap = -(n + 1000)
+                continue
+            _c(self, -1)
+
+    def bvss_discover(self, frontiers, mode=0, dense_min=4):
+        """Deviation edges leaving each frontier (a list of source-vertex
+        arrays); returns sorted (frontier, u, v) arrays."""
+        fptr = np.zeros(len(frontiers) + 1, np.int32)
+        for i, fr in enumerate(frontiers):
+            fptr[i + 1] = fptr[i] + len(fr)
+        fsrc = np.concatenate([np.asarray(fr, np.int32) for fr in frontiers]) if frontiers \
+            else np.zeros(0, np.int32)
+        fsrc = np.ascontiguousarray(fsrc if len(fsrc) else np.zeros(1, np.int32), np.int32)
+        cap = 1 << 16
+        while True:
+            of = np.zeros(cap, np.int32)
+            ou = np.zeros(cap, np.int32)
+            ov = np.zeros(cap, np.int32)
+            n = self.L.cpg_bvss_discover(self.h, len(frontiers), _iptr(fptr), _iptr(fsrc), int(mode),
+                                         int(dense_min), _iptr(of), _iptr(ou), _iptr(ov), cap)
+            if n >= 0:
+                return of[:n], ou[:n], ov[:n]
+            if n <= -1000:
+                cap = -(n + 1000)
+                continue
+            _c(self, -1)
 
     # -- static cache (gpucpg/gpucpg.cuh:186-189)
     def enable_tc_pfxt_static_cache(self, enabled):

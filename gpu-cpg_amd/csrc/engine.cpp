@@ -135,6 +135,23 @@ struct cpg_handle {
   std::vector<int> h_level, h_from, h_to, h_parent;
   bool host_result = false;
 
+  // BVSS deviation index (host copy for decoding, device copy for discovery)
+  struct HostBvss {
+    bool valid = false;
+    int n_int = 0;
+    std::vector<int> int_ptr, rows, entry_int;
+    std::vector<unsigned long long> masks;
+    std::vector<int2> tiles;
+    uint64_t set_bits = 0;
+  } hb;
+  DevBuf<int> bv_int_ptr, bv_rows, bv_entry_int, bv_fptr, bv_fsrc;
+  DevBuf<unsigned long long> bv_masks, bv_front;
+  DevBuf<int2> bv_tiles;
+  DevBuf<uint8_t> bv_dense;
+  DevBuf<int4> bv_out;
+  DevBuf<BvssCtl> bv_ctl;
+  EventPair ev_bvss;
+
   cpg_timing timing{};
   bool kernel_timing = false;
   EventPair ev_sfxt, ev_levels, ev_count, ev_fill;
@@ -167,6 +184,7 @@ struct cpg_handle {
 
   void invalidate_cache_payload() {
     sfxt_valid = false;
+    hb.valid = false;
     h_succ.clear();
   }
 
@@ -611,6 +629,109 @@ struct cpg_handle {
     host_result = true;
   }
 
+  // A_dev index: one entry per (64-source interval, destination row) with
+  // a deviation edge, entries ordered by interval then row.
+  void bvss_build() {
+    if (!sfxt_valid) run_sfxt(false);
+    fetch_succ();
+    const int n = g.n;
+    HostBvss b;
+    b.n_int = (n + 63) / 64;
+    std::vector<std::pair<uint64_t, int>> kb;  // {interval << 32 | row, source bit}
+    kb.reserve(static_cast<size_t>(g.m));
+    for (int u = 0; u < n; ++u)
+      for (int e = g.oadjp[u]; e < g.oadjp[u + 1]; ++e) {
+        const int v = g.oadj[e];
+        if (v == h_succ[u]) continue;
+        kb.emplace_back((static_cast<uint64_t>(u >> 6) << 32) | static_cast<uint32_t>(v), u & 63);
+      }
+    std::sort(kb.begin(), kb.end());
+    b.int_ptr.assign(static_cast<size_t>(b.n_int) + 1, 0);
+    for (size_t i = 0; i < kb.size();) {
+      const uint64_t key = kb[i].first;
+      unsigned long long m = 0;
+      for (; i < kb.size() && kb[i].first == key; ++i) m |= 1ull << kb[i].second;
+      const int iv = static_cast<int>(key >> 32);
+      b.rows.push_back(static_cast<int>(key & 0xffffffffu));
+      b.entry_int.push_back(iv);
+      b.masks.push_back(m);
+      b.set_bits += static_cast<uint64_t>(__builtin_popcountll(m));
+      b.int_ptr[static_cast<size_t>(iv) + 1]++;
+    }
+    for (int i = 0; i < b.n_int; ++i) b.int_ptr[i + 1] += b.int_ptr[i];
+    for (int i = 0; i < b.n_int; ++i)
+      for (int e = b.int_ptr[i]; e < b.int_ptr[i + 1]; e += 16) b.tiles.push_back(make_int2(i, e));
+    b.valid = true;
+    const size_t ne = b.rows.size();
+    bv_int_ptr.reserve(b.int_ptr.size()); bv_rows.reserve(std::max<size_t>(1, ne));
+    bv_entry_int.reserve(std::max<size_t>(1, ne)); bv_masks.reserve(std::max<size_t>(1, ne));
+    bv_tiles.reserve(std::max<size_t>(1, b.tiles.size())); bv_dense.reserve(std::max(1, b.n_int));
+    h2d(bv_int_ptr.p, b.int_ptr.data(), b.int_ptr.size(), stream);
+    h2d(bv_rows.p, b.rows.data(), ne, stream);
+    h2d(bv_entry_int.p, b.entry_int.data(), ne, stream);
+    h2d(bv_masks.p, b.masks.data(), ne, stream);
+    h2d(bv_tiles.p, b.tiles.data(), b.tiles.size(), stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    hb = std::move(b);
+  }
+
+  int64_t bvss_discover(int nfront, const int* fptr, const int* fsrc, int mode, int dense_min,
+                        int* of, int* ou, int* ov, int64_t cap) {
+    if (nfront < 0 || (nfront > 0 && (!fptr || (fptr[nfront] > 0 && !fsrc))))
+      throw std::runtime_error("bad frontier arguments");
+    if (mode < 0 || mode > 2) throw std::runtime_error("mode must be 0, 1 or 2");
+    if (cap < 0) throw std::runtime_error("cap must be >= 0");
+    if (!hb.valid) bvss_build();
+    timing.bvss_ms = 0.0;
+    timing.bvss_tiles = 0;
+    if (nfront == 0) return 0;
+    for (int f = 0; f < nfront; ++f)
+      if (fptr[f] > fptr[f + 1] || fptr[f] < 0) throw std::runtime_error("fptr must be ascending");
+    const int nsrc = fptr[nfront];
+    bv_fptr.reserve(static_cast<size_t>(nfront) + 1);
+    bv_fsrc.reserve(std::max(1, nsrc));
+    h2d(bv_fptr.p, fptr, static_cast<size_t>(nfront) + 1, stream);
+    h2d(bv_fsrc.p, fsrc, static_cast<size_t>(nsrc), stream);
+    const size_t fw = static_cast<size_t>(nfront) * std::max(1, hb.n_int);
+    bv_front.reserve(fw);
+    CPG_HIP(hipMemsetAsync(bv_front.p, 0, fw * sizeof(unsigned long long), stream));
+    bv_out.reserve(std::max<int64_t>(1, cap));
+    bv_ctl.reserve(1);
+    CPG_HIP(hipMemsetAsync(bv_ctl.p, 0, sizeof(BvssCtl), stream));
+    BvssArgs a{};
+    a.n = g.n; a.n_int = hb.n_int; a.n_entries = static_cast<int>(hb.rows.size());
+    a.n_tiles = static_cast<int>(hb.tiles.size()); a.nfront = nfront;
+    a.int_ptr = bv_int_ptr.p; a.rows = bv_rows.p; a.entry_int = bv_entry_int.p;
+    a.masks = bv_masks.p; a.tiles = bv_tiles.p; a.front = bv_front.p; a.dense = bv_dense.p;
+    a.out = bv_out.p; a.cap = static_cast<uint64_t>(cap); a.ctl = bv_ctl.p;
+    ev_bvss.create();
+    CPG_HIP(hipEventRecord(ev_bvss.a, stream));
+    launch_bvss_discover(a, bv_fptr.p, bv_fsrc.p, mode, dense_min, grid, stream);
+    CPG_HIP(hipEventRecord(ev_bvss.b, stream));
+    BvssCtl c{};
+    d2h(&c, bv_ctl.p, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    timing.bvss_ms = ev_bvss.ms();
+    timing.bvss_tiles = static_cast<int64_t>(c.tiles);
+    if (c.mismatch) throw std::runtime_error("BVSS MFMA count disagrees with the decoded words");
+    const uint64_t cnt = c.count;
+    if (cnt > static_cast<uint64_t>(cap)) return -static_cast<int64_t>(cnt) - 1000;
+    std::vector<int4> t(cnt);
+    d2h(t.data(), bv_out.p, cnt, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    std::sort(t.begin(), t.end(), [](const int4& x, const int4& y) {
+      if (x.x != y.x) return x.x < y.x;
+      if (x.y != y.y) return x.y < y.y;
+      return x.z < y.z;
+    });
+    for (uint64_t i = 0; i < cnt; ++i) {
+      if (of) of[i] = t[i].x;
+      if (ou) ou[i] = t[i].y;
+      if (ov) ov[i] = t[i].z;
+    }
+    return static_cast<int64_t>(cnt);
+  }
+
   void fetch_succ() {
     if (!h_succ.empty() || g.n == 0) return;
     if (!sfxt_valid) run_sfxt(false);
@@ -832,6 +953,58 @@ int cpg_get_sfxt(cpg_handle* h, int* dist_out, int* succ_out, int* level_out) {
     if (level_out) d2h(level_out, h->level.p, n, h->stream);
     CPG_HIP(hipStreamSynchronize(h->stream));
     return 0;
+  });
+}
+
+int cpg_bvss_build(cpg_handle* h) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (!h->loaded) throw std::runtime_error("no graph loaded");
+    h->bvss_build();
+    return 0;
+  });
+}
+
+int cpg_bvss_info(cpg_handle* h, int64_t* n_intervals, int64_t* n_entries, int64_t* set_bits) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (!h->hb.valid) throw std::runtime_error("BVSS index not built");
+    if (n_intervals) *n_intervals = h->hb.n_int;
+    if (n_entries) *n_entries = static_cast<int64_t>(h->hb.rows.size());
+    if (set_bits) *set_bits = static_cast<int64_t>(h->hb.set_bits);
+    return 0;
+  });
+}
+
+int64_t cpg_bvss_row_sources(cpg_handle* h, int row, int* out, int64_t cap) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int64_t>(-1, [&]() -> int64_t {
+    if (!h->hb.valid) throw std::runtime_error("BVSS index not built");
+    if (row < 0 || row >= h->g.n) throw std::runtime_error("row out of range");
+    std::vector<int> srcs;
+    for (size_t e = 0; e < h->hb.rows.size(); ++e) {
+      if (h->hb.rows[e] != row) continue;
+      unsigned long long m = h->hb.masks[e];
+      while (m) {
+        const int bit = __builtin_ctzll(m);
+        srcs.push_back(h->hb.entry_int[e] * 64 + bit);
+        m &= m - 1;
+      }
+    }
+    std::sort(srcs.begin(), srcs.end());
+    const int64_t len = static_cast<int64_t>(srcs.size());
+    if (len > cap) return -len - 1000;
+    if (out) std::copy(srcs.begin(), srcs.end(), out);
+    return len;
+  });
+}
+
+int64_t cpg_bvss_discover(cpg_handle* h, int nfront, const int* fptr, const int* fsrc, int mode,
+                          int dense_min, int* out_f, int* out_u, int* out_v, int64_t cap) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int64_t>(-1, [&]() -> int64_t {
+    if (!h->loaded) throw std::runtime_error("no graph loaded");
+    return h->bvss_discover(nfront, fptr, fsrc, mode, dense_min, out_f, out_u, out_v, cap);
   });
 }
 

@@ -130,6 +130,37 @@ void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s);
 void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
                          hipStream_t s);
 
+// ---------------- BVSS deviation discovery ----------------
+struct BvssCtl {
+  unsigned long long count;   // triples found (written while < cap)
+  unsigned long long tiles;   // MFMA tiles issued
+  int mismatch;               // an MFMA count disagreed with the decoded words
+  int pad;
+};
+
+struct BvssArgs {
+  int n;                      // vertices
+  int n_int;                  // 64-source intervals
+  int n_entries;              // (interval, destination row) entries
+  int n_tiles;                // 16-entry tiles (per interval, rounded up)
+  int nfront;                 // frontiers in this call
+  const int* int_ptr;         // n_int + 1: first entry of each interval
+  const int* rows;            // destination row of each entry
+  const int* entry_int;       // interval of each entry
+  const unsigned long long* masks;  // 64 source bits of each entry
+  const int2* tiles;          // {interval, first entry}
+  unsigned long long* front;  // nfront x n_int frontier words (zeroed by the caller)
+  uint8_t* dense;             // n_int: interval takes the MFMA path
+  int4* out;                  // {frontier, u, v, 0} triples
+  uint64_t cap;
+  BvssCtl* ctl;
+};
+
+// Frontier words, interval classification (mode 0 auto, 1 bitmap, 2 MFMA)
+// and both discovery paths.
+void launch_bvss_discover(const BvssArgs& a, const int* fptr, const int* fsrc, int mode,
+                          int dense_min, int grid, hipStream_t s);
+
 // ---------------- final selection ----------------
 // Stable LSD radix sort of (key, value) pairs over the given key bytes.
 struct SortBufs {

@@ -78,7 +78,9 @@ typedef struct {
   int64_t walk_launches;  /* kernel_timing: walk kernel launches, both passes            */
   double sfxt_levels_ms;  /* kernel_timing: SFXT level-loop kernel (part of sfxt_kernel_ms)  */
   int64_t sfxt_blocks;    /* SFXT level-loop grid used                                    */
-  int64_t probes;         /* bound-probe count passes run before the first round           */
+  int64_t probes;         /* bound-probe fill passes run before the first round            */
+  double bvss_ms;         /* last cpg_bvss_discover: device time (frontier build + both paths) */
+  int64_t bvss_tiles;     /* last cpg_bvss_discover: 16x16x64 int8 MFMA tiles issued        */
 } cpg_timing;
 
 /* ---- lifetime ---- */
@@ -127,6 +129,29 @@ int cpg_timings(cpg_handle* h, cpg_timing* out);
  * (bfs_td_step_privatized, gpucpg.cu:2215-2283).  Runs the SFXT if needed. */
 int cpg_sfxt(cpg_handle* h);
 int cpg_get_sfxt(cpg_handle* h, int* dist, int* succ, int* level);
+
+/* ---- BVSS deviation discovery, gpucpg/tc_pfxt_bvss.cuh ---- */
+/* Builds the A_dev index from the fanout CSR and the SFXT successors
+ * (build_adev_bvss_from_fanout_csr, tc_pfxt_bvss.cuh:117-176): for every
+ * 64-source interval, the destination rows v with a deviation edge u->v
+ * (v != succ[u]) from it and a 64-bit mask of those sources.  Runs the
+ * SFXT if needed; kept until the graph changes. */
+int cpg_bvss_build(cpg_handle* h);
+/* Index shape: intervals, (interval, row) entries, set source bits. */
+int cpg_bvss_info(cpg_handle* h, int64_t* n_intervals, int64_t* n_entries, int64_t* set_bits);
+/* Sources u with a deviation edge u->row, ascending (decode_row_neighbors,
+ * tc_pfxt_bvss.cuh:228-251); same return convention as
+ * cpg_get_path_vertices. */
+int64_t cpg_bvss_row_sources(cpg_handle* h, int row, int* out, int64_t cap);
+/* The deviation edges leaving each of nfront source frontiers
+ * (discover_pairs_for_sources, tc_pfxt_bvss.cuh:289-439, :572-640):
+ * frontier f is fsrc[fptr[f] .. fptr[f+1]).  Writes (f, u, v) triples
+ * sorted by (f, u, v) and returns their number (-(number) - 1000 when cap
+ * is too small).  mode 0: each interval takes the int8 MFMA path when its
+ * frontier block holds at least dense_min source bits per frontier on
+ * average, else the wave64 bitmap path; 1: bitmap only; 2: MFMA only. */
+int64_t cpg_bvss_discover(cpg_handle* h, int nfront, const int* fptr, const int* fsrc, int mode,
+                          int dense_min, int* out_f, int* out_u, int* out_v, int64_t cap);
 
 /* ---- static cache, gpucpg/gpucpg.cuh:186-189 ---- */
 int cpg_cache_enable(cpg_handle* h, int enabled);

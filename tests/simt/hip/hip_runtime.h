@@ -68,6 +68,9 @@ inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
 inline unsigned __float_as_uint(float f) { unsigned i; std::memcpy(&i, &f, 4); return i; }
 inline float __int_as_float(int i) { float f; std::memcpy(&f, &i, 4); return f; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline unsigned long long atomicOr(unsigned long long* p, unsigned long long v) {
+  return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
+}
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __lane_id() { return simt::lane_id(); }
 inline uint64_t __lanemask_lt() {
@@ -113,6 +116,42 @@ inline T __shfl_xor(T v, int m, int = 64) {
   return simt_from_bits<T>(s[(simt::lane_id() ^ m) & 63]);
 }
 inline void __syncthreads() { simt::block_sync(); }
+
+// v_mfma_i32_16x16x64_i8 as documented for CDNA wave64: lane l holds A
+// row l % 16 and B column l % 16, bytes 0..15 over k = 16 (l / 16) + j;
+// the result of lane l is D[4 (l / 16) + i][l % 16], i = 0..3.
+typedef int simt_v4i __attribute__((ext_vector_type(4)));
+inline simt_v4i __builtin_amdgcn_mfma_i32_16x16x64_i8(simt_v4i a, simt_v4i b, simt_v4i c, int, int,
+                                                      int) {
+  uint64_t part[4] = {
+      (static_cast<uint64_t>(static_cast<uint32_t>(a.y)) << 32) | static_cast<uint32_t>(a.x),
+      (static_cast<uint64_t>(static_cast<uint32_t>(a.w)) << 32) | static_cast<uint32_t>(a.z),
+      (static_cast<uint64_t>(static_cast<uint32_t>(b.y)) << 32) | static_cast<uint32_t>(b.x),
+      (static_cast<uint64_t>(static_cast<uint32_t>(b.w)) << 32) | static_cast<uint32_t>(b.z)};
+  int8_t A[64][16], B[64][16];
+  for (int p = 0; p < 4; ++p) {
+    const uint64_t* all = simt::wave_sync(part[p]);
+    for (int l = 0; l < 64; ++l) {
+      for (int j = 0; j < 8; ++j) {
+        const int8_t v = static_cast<int8_t>((all[l] >> (8 * j)) & 0xff);
+        if (p == 0) A[l][j] = v;
+        else if (p == 1) A[l][8 + j] = v;
+        else if (p == 2) B[l][j] = v;
+        else B[l][8 + j] = v;
+      }
+    }
+  }
+  const int l = simt::lane_id();
+  simt_v4i d = c;
+  for (int i = 0; i < 4; ++i) {
+    const int r = 4 * (l / 16) + i, col = l % 16;
+    int sum = 0;
+    for (int g = 0; g < 4; ++g)
+      for (int j = 0; j < 16; ++j) sum += static_cast<int>(A[g * 16 + r][j]) * static_cast<int>(B[g * 16 + col][j]);
+    d[i] += sum;
+  }
+  return d;
+}
 // every live lane of the wave meets here, as the wave does in lockstep
 inline void __builtin_amdgcn_wave_barrier() { (void)simt::wave_sync(0); }
 

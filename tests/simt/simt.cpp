@@ -182,7 +182,7 @@ void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, bool coo
     auto worker = [&](unsigned first, unsigned step) {
       Block B;
       for (unsigned b = first; b < nb; b += step) {
-        B.idx = dim3(b, 0, 0);
+        B.idx = dim3(b % grid.x, (b / grid.x) % grid.y, b / (grid.x * grid.y));
         B.dim = block;
         B.grid = grid;
         B.body = &body;
@@ -201,7 +201,7 @@ void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, bool coo
   std::vector<std::thread> th;
   std::vector<Block> blocks(nb);
   for (unsigned b = 0; b < nb; ++b) {
-    blocks[b].idx = dim3(b, 0, 0);
+    blocks[b].idx = dim3(b % grid.x, (b / grid.x) % grid.y, b / (grid.x * grid.y));
     blocks[b].dim = block;
     blocks[b].grid = grid;
     blocks[b].body = &body;
