@@ -79,6 +79,13 @@ ABI = {
     "cpg_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "cpg_sfxt": (ctypes.c_int, [_H]),
     "cpg_get_sfxt": (ctypes.c_int, [_H, _PI, _PI, _PI]),
+    "cpg_batch_create": (_H, []),
+    "cpg_batch_destroy": (None, [_H]),
+    "cpg_batch_add_edges": (ctypes.c_int, [_H, ctypes.c_int, _I64, _PI, _PI, _PF, _I64]),
+    "cpg_batch_num_queries": (ctypes.c_int, [_H]),
+    "cpg_batch_run": (ctypes.c_int, [_H, ctypes.POINTER(QueryOpts)]),
+    "cpg_batch_get_costs": (_I64, [_H, ctypes.c_int, _I64, _PF]),
+    "cpg_batch_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "cpg_bvss_build": (ctypes.c_int, [_H]),
     "cpg_bvss_info": (ctypes.c_int, [_H, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                      ctypes.POINTER(_I64)]),
@@ -347,3 +354,46 @@ class CpGen:
 
     def tc_pfxt_static_cache_misses(self):
         return self.L.cpg_cache_misses(self.h)
+
+
+class Batch:
+    """Many designs, one K-path query each, solved together (cpg_batch_*)."""
+
+    def __init__(self, library=None):
+        self.L = library or lib()
+        self.h = self.L.cpg_batch_create()
+        if not self.h:
+            raise RuntimeError(self.L.cpg_last_error().decode())
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.cpg_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def add(self, n, u, v, w, k):
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        w = np.ascontiguousarray(w, np.float32)
+        return _check(self.L.cpg_batch_add_edges(self.h, int(n), len(u), _iptr(u), _iptr(v),
+                                                 _fptr(w), int(k)), self.L)
+
+    def run(self, root_tie_quirk=True, init_split_perc=0.005, sfxt_blocks=0):
+        o = QueryOpts()
+        self.L.cpg_query_opts_default(ctypes.byref(o))
+        o.root_tie_quirk = 1 if root_tie_quirk else 0
+        o.init_split_perc = float(init_split_perc)
+        o.sfxt_blocks = int(sfxt_blocks)
+        _check(self.L.cpg_batch_run(self.h, ctypes.byref(o)), self.L)
+
+    def get_slacks(self, q, k):
+        out = np.zeros(max(int(k), 1), np.float32)
+        n = _check(self.L.cpg_batch_get_costs(self.h, int(q), int(k), _fptr(out)), self.L)
+        return out[:n]
+
+    def timings(self):
+        t = Timing()
+        _check(self.L.cpg_batch_timings(self.h, ctypes.byref(t)), self.L)
+        return t.as_dict()

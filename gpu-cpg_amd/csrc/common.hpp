@@ -50,12 +50,15 @@ __host__ __device__ inline uint32_t bin_upper_key(uint32_t b, uint32_t klo, uint
   const uint64_t u = static_cast<uint64_t>(klo) + ((static_cast<uint64_t>(b) + 1) << shift) - 1;
   return u > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(u);
 }
-// Smallest shift whose window covers [klo, khi].
-__host__ __device__ inline uint32_t window_shift(uint32_t klo, uint32_t khi) {
+// Smallest shift whose window of `bins` bins covers [klo, khi].
+__host__ __device__ inline uint32_t window_shift_bins(uint32_t klo, uint32_t khi, int bins) {
   uint32_t s = 0;
   const uint64_t span = khi >= klo ? static_cast<uint64_t>(khi - klo) : 0;
-  while ((span >> s) >= static_cast<uint64_t>(kHistBins)) ++s;
+  while ((span >> s) >= static_cast<uint64_t>(bins)) ++s;
   return s;
+}
+__host__ __device__ inline uint32_t window_shift(uint32_t klo, uint32_t khi) {
+  return window_shift_bins(klo, khi, kHistBins);
 }
 
 __device__ inline uint64_t lanemask_lt() { return __lanemask_lt(); }

@@ -31,6 +31,12 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Wall-clock limit of one query's PFXT rounds; a build that emulates the
+// kernels on the CPU raises it.
+#ifndef CPG_QUERY_LIMIT_MS
+#define CPG_QUERY_LIMIT_MS 120000.0
+#endif
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
              std::chrono::steady_clock::now().time_since_epoch())
@@ -507,7 +513,7 @@ struct cpg_handle {
       int r = 0;
       while (true) {
         if (r > 100000) throw std::runtime_error("PFXT round limit exceeded");
-        if (now_ms() - tq0 > 120000.0) throw std::runtime_error("PFXT query exceeded 120 s");
+        if (now_ms() - tq0 > CPG_QUERY_LIMIT_MS) throw std::runtime_error("PFXT query exceeded its time limit");
         // Once the bound is finite and set from stored paths, every round is
         // the same fill-only sequence and every kernel does nothing after
         // the done flag is set: launch a group of rounds per host sync.
@@ -738,6 +744,254 @@ struct cpg_handle {
     h_succ.resize(g.n);
     d2h(h_succ.data(), succ.p, g.n, stream);
     CPG_HIP(hipStreamSynchronize(stream));
+  }
+};
+
+// Batched queries: the disjoint union of several designs, one top-K query
+// each, expanded together by the multi-query PFXT (mq.hip).  The union's
+// SFXT is every design's own SFXT, so one level loop serves all of them;
+// the rounds after the first are captured once in a HIP graph and
+// replayed.
+struct cpg_batch {
+  struct Design {
+    int n = 0;
+    std::vector<int> u, v;
+    std::vector<float> w;
+    int64_t k = 0;
+  };
+  std::vector<Design> designs;
+  std::vector<int> voff;              // first union vertex of each design
+  cpg_handle uh;                      // the union graph (CSR + SFXT)
+  bool built = false;
+
+  DevBuf<int> nq, root_v, root_q;
+  DevBuf<MqQuery> qs;
+  DevBuf<MqCtl> mctl;
+  DevBuf<unsigned> hist;
+  DevBuf<float> n_cost;
+  DevBuf<int> n_to, n_from, n_parent, n_level;
+  DevBuf<uint32_t> n_key, batch;
+  DevBuf<uint8_t> n_flag;
+  DevBuf<uint64_t> skey[2];
+  DevBuf<uint32_t> sval[2];
+  DevBuf<unsigned> ghist;
+  DevBuf<unsigned long long> nsel, per_q;
+  DevBuf<float> out_cost;
+  uint64_t cap = 0;
+
+  std::vector<std::vector<float>> results;
+  cpg_timing timing{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  MqArgs graph_args{};   // the arguments the captured round was recorded with
+  int graph_grid = 0;
+
+  ~cpg_batch() { drop_graph(); }
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
+
+  void build_union() {
+    EdgeList el;
+    voff.assign(designs.size() + 1, 0);
+    size_t m = 0;
+    for (size_t q = 0; q < designs.size(); ++q) {
+      voff[q + 1] = voff[q] + designs[q].n;
+      m += designs[q].u.size();
+    }
+    el.n = voff.back();
+    el.u.reserve(m); el.v.reserve(m); el.w.reserve(m);
+    for (size_t q = 0; q < designs.size(); ++q) {
+      const Design& d = designs[q];
+      for (size_t e = 0; e < d.u.size(); ++e) {
+        el.u.push_back(d.u[e] + voff[q]);
+        el.v.push_back(d.v[e] + voff[q]);
+        el.w.push_back(d.w[e]);
+      }
+    }
+    uh.set_graph(el);
+    built = true;
+  }
+
+  void ensure(uint64_t c, int nqc) {
+    n_cost.reserve(c); n_to.reserve(c); n_from.reserve(c); n_parent.reserve(c);
+    n_level.reserve(c); n_key.reserve(c); n_flag.reserve(c); nq.reserve(c); batch.reserve(c);
+    qs.reserve(nqc); mctl.reserve(1);
+    hist.reserve(static_cast<size_t>(nqc) * kMqHistStride);
+    cap = c;
+  }
+
+  void run(const cpg_query_opts& o) {
+    if (designs.empty()) throw std::runtime_error("no designs in the batch");
+    if (!built) build_union();
+    timing = cpg_timing{};
+    const double t0 = now_ms();
+    uh.kernel_timing = false;
+    uh.sfxt_blocks = o.sfxt_blocks;
+    uh.reset();
+    timing.sfxt_ms = uh.run_sfxt(false);
+    hipStream_t stream = uh.stream;
+    const double tp0 = now_ms();
+    const int nqc = static_cast<int>(designs.size());
+    // roots per design (ascending id), with each design's own initial-split quirk
+    const int nsrc = static_cast<int>(uh.g.srcs.size());
+    std::vector<int> sd(nsrc);
+    uh.src_dist.reserve(std::max(1, nsrc));
+    launch_gather_int(uh.dist.p, uh.srcs.p, nsrc, uh.src_dist.p, stream);
+    d2h(sd.data(), uh.src_dist.p, nsrc, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    std::vector<std::vector<int>> rv(nqc);
+    std::vector<std::vector<float>> rcost(nqc);
+    for (int i = 0; i < nsrc; ++i) {
+      if (sd[i] == kIntMax) continue;
+      const int s = uh.g.srcs[i];
+      const int q = static_cast<int>(std::upper_bound(voff.begin(), voff.end(), s) - voff.begin()) - 1;
+      rv[q].push_back(s);
+      rcost[q].push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+    }
+    std::vector<int> root_vh, root_qh;
+    std::vector<MqQuery> qh(nqc);
+    uint64_t need = 0;
+    for (int q = 0; q < nqc; ++q) {
+      std::vector<uint8_t> keep(rv[q].size(), 1);
+      const size_t nr = rv[q].size();
+      if (o.root_tie_quirk && nr > 0) {
+        std::vector<float> sorted = rcost[q];
+        std::sort(sorted.begin(), sorted.end());
+        const float pn = o.init_split_perc * static_cast<float>(nr);
+        const float split = sorted[static_cast<size_t>(pn)];
+        const int num_short = static_cast<int>(pn + 1);
+        int kept = 0;
+        for (size_t i = 0; i < nr; ++i) {
+          if (rcost[q][i] > split) continue;
+          if (kept < num_short) ++kept;
+          else keep[i] = 0;
+        }
+      }
+      float minc = __builtin_huge_valf();
+      int cnt = 0;
+      for (size_t i = 0; i < nr; ++i)
+        if (keep[i]) {
+          root_vh.push_back(rv[q][i]);
+          root_qh.push_back(q);
+          minc = std::min(minc, rcost[q][i]);
+          ++cnt;
+        }
+      MqQuery& m = qh[q];
+      m = MqQuery{};
+      m.ukey = kKeyInf;
+      m.klo = cnt ? float_key(minc) : 0u;
+      m.shift = window_shift_bins(m.klo, kKeyInf, kMqBins);
+      m.k = static_cast<unsigned long long>(std::max<int64_t>(0, designs[q].k));
+      m.batch_target = std::max<unsigned long long>(256, m.k / 8);
+      m.done = (cnt == 0 || m.k == 0) ? 1 : 0;
+      need += 16ull * m.k + static_cast<uint64_t>(cnt) + 4096;
+      timing.roots += cnt;
+      timing.roots_dropped += static_cast<int64_t>(nr) - cnt;
+    }
+    const int nroots = static_cast<int>(root_vh.size());
+    root_v.reserve(std::max(1, nroots)); root_q.reserve(std::max(1, nroots));
+    h2d(root_v.p, root_vh.data(), nroots, stream);
+    h2d(root_q.p, root_qh.data(), nroots, stream);
+    uint64_t c = std::min<uint64_t>(std::max<uint64_t>(need, 1u << 20), 0x7fffffffull);
+    MqCtl mc{};
+    for (int attempt = 0;; ++attempt) {
+      ensure(c, nqc);
+      h2d(qs.p, qh.data(), nqc, stream);
+      CPG_HIP(hipMemsetAsync(mctl.p, 0, sizeof(MqCtl), stream));
+      MqArgs a = args(nqc);
+      launch_mq_roots(a, root_v.p, root_q.p, nroots, stream);
+      bool overflow = false;
+      for (int r = 0;; ++r) {
+        if (r > 100000) throw std::runtime_error("batched PFXT round limit exceeded");
+        if (now_ms() - t0 > CPG_QUERY_LIMIT_MS) throw std::runtime_error("batched query exceeded its time limit");
+        // round 0 expands every root; later rounds replay the captured round
+        const int group = r == 0 ? 1 : 4;
+        if (r == 0) {
+          launch_mq_round(a, true, uh.grid, stream);
+        } else {
+          // the captured round holds buffer addresses and sizes: capture
+          // again whenever any of them changed (node store growth, designs
+          // added, a rebuilt union graph)
+          if (graph_exec && (std::memcmp(&graph_args, &a, sizeof(MqArgs)) != 0 || graph_grid != uh.grid))
+            drop_graph();
+          if (!graph_exec) {
+            CPG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            launch_mq_round(a, false, uh.grid, stream);
+            CPG_HIP(hipStreamEndCapture(stream, &graph));
+            CPG_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+            graph_args = a;
+            graph_grid = uh.grid;
+          }
+          for (int gi = 0; gi < group; ++gi) CPG_HIP(hipGraphLaunch(graph_exec, stream));
+        }
+        d2h(&mc, mctl.p, 1, stream);
+        CPG_HIP(hipStreamSynchronize(stream));
+        timing.rounds += group;
+        if (mc.err) throw std::runtime_error("batched PFXT walk exceeded its step bound");
+        if (mc.overflow) { overflow = true; break; }
+        if (mc.active == 0) break;
+      }
+      if (!overflow) break;
+      if (attempt >= 4 || c >= 0x7fffffffull) throw std::runtime_error("prefix-tree node capacity exceeded");
+      c = std::min<uint64_t>(c * 2, 0x7fffffffull);
+      timing.retries++;
+    }
+    timing.generated = static_cast<int64_t>(mc.node_n);
+    const double ts0 = now_ms();
+    timing.pfxt_ms = ts0 - tp0;
+    // final selection: (query, cost, level) sort of the paths at or below
+    // each query's bound; each query's first k are contiguous
+    const uint64_t nn = std::min<uint64_t>(mc.node_n, cap);
+    skey[0].reserve(std::max<uint64_t>(1, nn)); skey[1].reserve(std::max<uint64_t>(1, nn));
+    sval[0].reserve(std::max<uint64_t>(1, nn)); sval[1].reserve(std::max<uint64_t>(1, nn));
+    nsel.reserve(1); per_q.reserve(nqc);
+    CPG_HIP(hipMemsetAsync(nsel.p, 0, sizeof(unsigned long long), stream));
+    CPG_HIP(hipMemsetAsync(per_q.p, 0, sizeof(unsigned long long) * nqc, stream));
+    MqArgs a = args(nqc);
+    launch_mq_build_keys(a, nn, skey[0].p, sval[0].p, nsel.p, per_q.p, uh.grid, stream);
+    unsigned long long nl = 0;
+    std::vector<unsigned long long> pq(nqc);
+    d2h(&nl, nsel.p, 1, stream);
+    d2h(pq.data(), per_q.p, nqc, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
+    SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
+    std::vector<int> bytes = {0, 1, 2, 3, 4, 5};
+    if (nqc > 1) bytes.push_back(6);
+    if (nqc > 256) bytes.push_back(7);
+    const int res = radix_sort_pairs(sb, nl, bytes.data(), static_cast<int>(bytes.size()), stream);
+    out_cost.reserve(std::max<uint64_t>(1, nl));
+    launch_mq_gather_costs(sval[res].p, n_cost.p, nl, out_cost.p, stream);
+    std::vector<float> all(nl);
+    d2h(all.data(), out_cost.p, nl, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    results.assign(nqc, {});
+    uint64_t off = 0;
+    for (int q = 0; q < nqc; ++q) {
+      const uint64_t take = std::min<uint64_t>(pq[q], qh[q].k);
+      results[q].assign(all.begin() + off, all.begin() + off + take);
+      off += pq[q];
+    }
+    const double te = now_ms();
+    timing.select_ms = te - ts0;
+    timing.query_ms = te - t0;
+  }
+
+  MqArgs args(int nqc) {
+    MqArgs a{};
+    a.nodes = NodeStore{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, n_key.p, n_flag.p, cap};
+    a.nq = nq.p;
+    a.vrec = uh.vrec.p; a.devinfo = uh.devinfo.p; a.devs = uh.devs.p; a.distf = uh.distf.p;
+    a.batch = batch.p; a.hist = hist.p; a.qs = qs.p; a.ctl = mctl.p; a.nq_count = nqc;
+    a.max_abs_distf = uh.max_abs_distf; a.max_abs_wgt = uh.g.max_abs_wgt;
+    a.step_limit = static_cast<int>(std::min<int64_t>(
+        0x3fffffff, (static_cast<int64_t>(uh.diameter) + 2) * (uh.g.max_out_degree + 2) + 16));
+    return a;
   }
 };
 
@@ -1006,6 +1260,68 @@ int64_t cpg_bvss_discover(cpg_handle* h, int nfront, const int* fptr, const int*
     if (!h->loaded) throw std::runtime_error("no graph loaded");
     return h->bvss_discover(nfront, fptr, fsrc, mode, dense_min, out_f, out_u, out_v, cap);
   });
+}
+
+cpg_batch* cpg_batch_create(void) {
+  return guarded<cpg_batch*>(nullptr, []() { return new cpg_batch(); });
+}
+
+void cpg_batch_destroy(cpg_batch* b) { delete b; }
+
+int cpg_batch_add_edges(cpg_batch* b, int n, int64_t m, const int* eu, const int* ev,
+                        const float* ew, int64_t k) {
+  if (!b) { g_last_error = "null batch"; return -1; }
+  return guarded<int>(-1, [&]() {
+    if (n < 0 || m < 0 || (m > 0 && (!eu || !ev || !ew))) throw std::runtime_error("bad edge list");
+    if (k < 0) throw std::runtime_error("k must be >= 0");
+    for (int64_t e = 0; e < m; ++e)
+      if (eu[e] < 0 || eu[e] >= n || ev[e] < 0 || ev[e] >= n) throw std::runtime_error("vertex id out of range");
+    cpg_batch::Design d;
+    d.n = n;
+    d.u.assign(eu, eu + m);
+    d.v.assign(ev, ev + m);
+    d.w.assign(ew, ew + m);
+    d.k = k;
+    b->designs.push_back(std::move(d));
+    b->built = false;
+    return static_cast<int>(b->designs.size()) - 1;
+  });
+}
+
+int cpg_batch_num_queries(cpg_batch* b) { return b ? static_cast<int>(b->designs.size()) : -1; }
+
+int cpg_batch_run(cpg_batch* b, const cpg_query_opts* o) {
+  if (!b) { g_last_error = "null batch"; return -1; }
+  return guarded<int>(-1, [&]() {
+    cpg_query_opts d;
+    cpg_query_opts_default(&d);
+    if (o) {
+      if (o->struct_size != static_cast<int>(sizeof(cpg_query_opts)))
+        throw std::runtime_error("cpg_query_opts.struct_size mismatch");
+      d = *o;
+    }
+    if (d.endpoint_mask) throw std::runtime_error("endpoint_mask is not supported in a batch");
+    b->run(d);
+    return 0;
+  });
+}
+
+int64_t cpg_batch_get_costs(cpg_batch* b, int q, int64_t k, float* out) {
+  if (!b) { g_last_error = "null batch"; return -1; }
+  return guarded<int64_t>(-1, [&]() -> int64_t {
+    if (q < 0 || q >= static_cast<int>(b->results.size())) throw std::runtime_error("no result for that query");
+    const auto& r = b->results[q];
+    const int64_t n = std::min<int64_t>(std::max<int64_t>(k, 0), static_cast<int64_t>(r.size()));
+    if (n > 0 && !out) throw std::runtime_error("null output");
+    std::copy(r.begin(), r.begin() + n, out);
+    return n;
+  });
+}
+
+int cpg_batch_timings(cpg_batch* b, cpg_timing* out) {
+  if (!b || !out) { g_last_error = "null argument"; return -1; }
+  *out = b->timing;
+  return 0;
 }
 
 int cpg_cache_enable(cpg_handle* h, int enabled) {

@@ -130,6 +130,59 @@ void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s);
 void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
                          hipStream_t s);
 
+// ---------------- multi-query PFXT (batched designs) ----------------
+constexpr int kMqBins = 1024;                 // per-query histogram bins
+constexpr int kMqHistStride = 3 * kMqBins;    // counts, per-bin max key, unexpanded counts
+
+struct MqQuery {
+  uint32_t ukey;                  // bound (kKeyInf = none)
+  uint32_t thkey;                 // batch threshold
+  uint32_t klo;                   // histogram window base key
+  uint32_t shift;                 // histogram window shift
+  unsigned long long k;           // paths asked for
+  unsigned long long batch_target;
+  unsigned long long expanded;
+  int done;
+  int rounds;
+};
+
+struct MqCtl {
+  unsigned long long node_n;
+  unsigned long long batch_n;
+  int overflow;
+  int err;
+  int active;                     // queries with unexpanded paths this round
+  int pad;
+};
+
+struct MqArgs {
+  NodeStore nodes;
+  int* nq;                        // query of each node
+  const int4* vrec;
+  const int2* devinfo;
+  const int4* devs;
+  const float* distf;
+  uint32_t* batch;
+  unsigned* hist;                 // nq_count x kMqHistStride
+  MqQuery* qs;
+  MqCtl* ctl;
+  int nq_count;
+  float max_abs_distf;
+  float max_abs_wgt;
+  int step_limit;
+};
+
+void launch_mq_roots(const MqArgs& a, const int* root_v, const int* root_q, int nroots, hipStream_t s);
+// One round for every query (all: the batch takes every unexpanded path).
+void launch_mq_round(const MqArgs& a, bool all, int grid, hipStream_t s);
+void launch_mq_build_keys(const MqArgs& a, uint64_t nmax, uint64_t* key, uint32_t* val,
+                          unsigned long long* count, unsigned long long* per_q, int grid,
+                          hipStream_t s);
+void launch_mq_gather_costs(const uint32_t* val, const float* cost, uint64_t n, float* out,
+                            hipStream_t s);
+void launch_zero_u32(unsigned* p, uint64_t n, hipStream_t s);
+void launch_zero_active(MqCtl* ctl, hipStream_t s);
+
 // ---------------- BVSS deviation discovery ----------------
 struct BvssCtl {
   unsigned long long count;   // triples found (written while < cap)

@@ -1,0 +1,353 @@
+// Multi-query PFXT: Q independent top-K queries expanded together.
+//
+// Every query q owns a set of roots and its own bound U_q, histogram
+// window and K_q; the node store, the batch and every kernel are shared,
+// and a node carries the id of its query.  The queries run on the disjoint
+// union of their graphs (batched designs), whose SFXT is each graph's own
+// SFXT, so one level loop serves all of them.  Per round, for every query
+// at once:
+//   1. histograms of its stored paths at or below U_q (global atomics into
+//      its own kMqBins bins: Q x kMqBins bins keep contention low),
+//   2. the batch threshold from its unexpanded paths below U_q,
+//   3. a count pass over the batch histogramming its children (< U_q),
+//   4. U_q tightened to the largest key of the bin where the running count
+//      of its stored paths and counted children reaches K_q,
+//   5. a fill pass storing its children <= U_q (inclusive: the counted
+//      children at U_q are needed).
+// The per-query semantics equal the single-query rounds' (see pfxt.hip);
+// the final selection sorts (query, cost, level) so each query's K_q
+// results are contiguous.
+#include <algorithm>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "walk.hpp"
+
+namespace cpg {
+
+namespace {
+
+constexpr int kMqBlock = 256;
+constexpr int kMqSelBlock = 256;
+
+__device__ inline uint32_t mq_bin(uint32_t k, uint32_t klo, uint32_t shift) {
+  if (k < klo) return 0;
+  const uint32_t b = (k - klo) >> shift;
+  return b < static_cast<uint32_t>(kMqBins) ? b : static_cast<uint32_t>(kMqBins - 1);
+}
+
+__device__ inline uint32_t mq_shift(uint32_t klo, uint32_t khi) {
+  uint32_t s = 0;
+  const uint64_t span = khi >= klo ? static_cast<uint64_t>(khi - klo) : 0;
+  while ((span >> s) >= static_cast<uint64_t>(kMqBins)) ++s;
+  return s;
+}
+
+// Block-wide first bin at which the running sum of h reaches target;
+// kMqBins if the total is smaller.  *total receives the sum.
+__device__ uint32_t mq_crossing(const unsigned* h, unsigned long long target,
+                                unsigned long long* total) {
+  constexpr int kPer = kMqBins / kMqSelBlock;
+  __shared__ unsigned long long s_part[kMqSelBlock];
+  __shared__ unsigned s_bin;
+  const int t = threadIdx.x;
+  unsigned long long mine = 0;
+  for (int i = 0; i < kPer; ++i) mine += h[t * kPer + i];
+  s_part[t] = mine;
+  if (t == 0) s_bin = kMqBins;
+  __syncthreads();
+  for (int off = 1; off < kMqSelBlock; off <<= 1) {
+    const unsigned long long v = t >= off ? s_part[t - off] : 0ull;
+    __syncthreads();
+    s_part[t] += v;
+    __syncthreads();
+  }
+  const unsigned long long incl = s_part[t];
+  const unsigned long long excl = incl - mine;
+  *total = s_part[kMqSelBlock - 1];
+  if (excl < target && incl >= target) {
+    unsigned long long run = excl;
+    for (int i = 0; i < kPer; ++i) {
+      run += h[t * kPer + i];
+      if (run >= target) { s_bin = static_cast<uint32_t>(t * kPer + i); break; }
+    }
+  }
+  __syncthreads();
+  const uint32_t b = s_bin;
+  __syncthreads();
+  return b;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(kMqBlock) mq_roots_kernel(MqArgs a, const int* root_v,
+                                                            const int* root_q, int nroots) {
+  for (int base = blockIdx.x * blockDim.x; base < nroots; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    const bool take = i < nroots;
+    const uint64_t slot = wave_reserve(&a.ctl->node_n, take);
+    if (take) {
+      const int s = root_v[i];
+      const float c = a.distf[s];
+      a.nodes.cost[slot] = c;
+      a.nodes.to[slot] = s;
+      a.nodes.from[slot] = -1;
+      a.nodes.parent[slot] = -1;
+      a.nodes.level[slot] = 0;
+      a.nodes.key[slot] = float_key(c);
+      a.nodes.flag[slot] = 0;
+      a.nq[slot] = root_q[i];
+    }
+  }
+}
+
+// Histograms of every stored path at or below its query's bound: nonzero
+// keys with each bin's largest key, and the unexpanded paths below the
+// bound.
+__global__ void __launch_bounds__(kMqBlock) mq_hist_kernel(MqArgs a) {
+  const uint64_t n = a.ctl->node_n < a.nodes.cap ? a.ctl->node_n : a.nodes.cap;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const int q = a.nq[i];
+    const MqQuery qs = a.qs[q];
+    const uint32_t k = a.nodes.key[i];
+    if (qs.done || k > qs.ukey) continue;
+    const uint32_t b = mq_bin(k, qs.klo, qs.shift);
+    unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
+    if (!key_is_zero(k)) {
+      atomicAdd(&h[b], 1u);
+      atomicMax(&h[kMqBins + b], k);
+    }
+    if (k < qs.ukey && a.nodes.flag[i] == 0) atomicAdd(&h[2 * kMqBins + b], 1u);
+  }
+}
+
+// One block per query: the batch threshold (target unexpanded paths below
+// the bound); a query with none left is done.
+__global__ void __launch_bounds__(kMqSelBlock) mq_select_batch_kernel(MqArgs a, int all) {
+  const int q = blockIdx.x;
+  MqQuery* qs = a.qs + q;
+  if (qs->done) return;
+  const unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
+  unsigned long long total = 0;
+  const uint32_t b = mq_crossing(h + 2 * kMqBins, all ? ~0ull : qs->batch_target, &total);
+  if (threadIdx.x == 0) {
+    if (total == 0) {
+      qs->done = 1;
+    } else if (b >= static_cast<uint32_t>(kMqBins)) {
+      qs->thkey = qs->ukey;
+    } else {
+      const uint64_t up = static_cast<uint64_t>(qs->klo) + ((static_cast<uint64_t>(b) + 1) << qs->shift) - 1;
+      const uint32_t upk = up > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(up);
+      qs->thkey = upk < qs->ukey ? upk : qs->ukey;
+    }
+    if (total != 0) atomicAdd(&a.ctl->active, 1);
+  }
+}
+
+__global__ void __launch_bounds__(kMqBlock) mq_make_batch_kernel(MqArgs a) {
+  const uint64_t n = a.ctl->node_n < a.nodes.cap ? a.ctl->node_n : a.nodes.cap;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    bool take = false;
+    if (i < n && a.nodes.flag[i] == 0) {
+      const MqQuery qs = a.qs[a.nq[i]];
+      const uint32_t k = a.nodes.key[i];
+      take = !qs.done && k <= qs.thkey && k < qs.ukey;
+    }
+    const uint64_t slot = wave_reserve(&a.ctl->batch_n, take);
+    if (take) {
+      a.nodes.flag[i] = 1;
+      a.batch[slot] = static_cast<uint32_t>(i);
+      atomicAdd(&a.qs[a.nq[i]].expanded, 1ull);
+    }
+  }
+}
+
+// Count pass (kFill false): histograms each query's children < U_q.
+// Fill pass: stores each query's children <= U_q.
+template <bool kFill>
+__global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
+  MqCtl* ctl = a.ctl;
+  const uint64_t nb = ctl->batch_n;
+  const float mag = 2.0f * a.max_abs_distf + a.max_abs_wgt + 1.0f;
+  unsigned long long hops = 0, devv = 0;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < nb;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    int p = 0, lvl = 0, q = 0, to = -1;
+    float c = 0.0f;
+    MqQuery qs{};
+    qs.ukey = kKeyInf;
+    if (i < nb) {
+      p = static_cast<int>(a.batch[i]);
+      c = a.nodes.cost[p];
+      to = a.nodes.to[p];
+      lvl = a.nodes.level[p];
+      q = a.nq[p];
+      qs = a.qs[q];
+    }
+    const float U = key_float(qs.ukey);
+    ChainWalk w;
+    w.start(i < nb, c, to, U, qs.ukey != kKeyInf, mag);
+    unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
+    while (__ballot(w.mode != 2)) {
+      float cc = 0.0f;
+      int cw = 0, from = 0;
+      const bool emit = w.step(a.vrec, a.devinfo, a.devs, U, !kFill, a.step_limit, &ctl->err,
+                               &hops, &devv, &cc, &cw, &from);
+      if (!kFill) {
+        if (emit) {
+          const uint32_t k = float_key(cc);
+          if (!key_is_zero(k)) {
+            const uint32_t b = mq_bin(k, qs.klo, qs.shift);
+            atomicAdd(&h[b], 1u);
+            atomicMax(&h[kMqBins + b], k);
+          }
+        }
+        continue;
+      }
+      const uint64_t slot = wave_reserve(&ctl->node_n, emit);
+      if (emit) {
+        if (slot < a.nodes.cap) {
+          a.nodes.cost[slot] = cc;
+          a.nodes.to[slot] = cw;
+          a.nodes.from[slot] = from;
+          a.nodes.parent[slot] = p;
+          a.nodes.level[slot] = lvl + 1;
+          a.nodes.key[slot] = float_key(cc);
+          a.nodes.flag[slot] = 0;
+          a.nq[slot] = q;
+        } else {
+          atomicExch(&ctl->overflow, 1);
+        }
+      }
+    }
+  }
+  (void)hops;
+  (void)devv;
+}
+
+// One block per query: U_q = the largest key of the bin where the running
+// count of stored paths and counted children reaches K_q.
+__global__ void __launch_bounds__(kMqSelBlock) mq_select_bound_kernel(MqArgs a) {
+  const int q = blockIdx.x;
+  MqQuery* qs = a.qs + q;
+  if (qs->done) return;
+  const unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
+  unsigned long long total = 0;
+  const uint32_t b = mq_crossing(h, qs->k, &total);
+  if (threadIdx.x == 0) {
+    uint32_t uk = qs->ukey;
+    if (b < static_cast<uint32_t>(kMqBins)) {
+      const uint32_t top = h[kMqBins + b];
+      if (top < uk) uk = top;
+    }
+    qs->ukey = uk;
+    qs->shift = mq_shift(qs->klo, uk);
+    qs->rounds += 1;
+  }
+}
+
+// Sort input: each stored path with a nonzero key <= its query's bound as
+// (query << 48 | key << 16 | level) -> node.
+__global__ void __launch_bounds__(kMqBlock) mq_build_keys_kernel(MqArgs a, uint64_t* key,
+                                                                 uint32_t* val,
+                                                                 unsigned long long* count,
+                                                                 unsigned long long* per_q) {
+  const uint64_t n = a.ctl->node_n < a.nodes.cap ? a.ctl->node_n : a.nodes.cap;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n;
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = base + threadIdx.x;
+    bool take = false;
+    uint32_t k = 0;
+    int q = 0;
+    if (i < n) {
+      q = a.nq[i];
+      k = a.nodes.key[i];
+      take = k <= a.qs[q].ukey && !key_is_zero(k);
+    }
+    const uint64_t slot = wave_reserve(count, take);
+    if (take) {
+      key[slot] = (static_cast<uint64_t>(q) << 48) | (static_cast<uint64_t>(k) << 16) |
+                  static_cast<uint32_t>(a.nodes.level[i] & 0xffff);
+      val[slot] = static_cast<uint32_t>(i);
+      atomicAdd(&per_q[q], 1ull);
+    }
+  }
+}
+
+__global__ void mq_gather_costs_kernel(const uint32_t* val, const float* cost, uint64_t n,
+                                       float* out) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    out[i] = cost[val[i]];
+}
+
+__global__ void zero_u32_kernel(unsigned* p, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ void zero_active_kernel(MqCtl* ctl) {
+  ctl->active = 0;
+  ctl->batch_n = 0;
+}
+
+void launch_zero_u32(unsigned* p, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  const uint64_t g = std::min<uint64_t>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(static_cast<unsigned>(g)), dim3(256), 0, s, p, n);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_zero_active(MqCtl* ctl, hipStream_t s) {
+  hipLaunchKernelGGL(zero_active_kernel, dim3(1), dim3(1), 0, s, ctl);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_mq_roots(const MqArgs& a, const int* root_v, const int* root_q, int nroots, hipStream_t s) {
+  if (nroots <= 0) return;
+  hipLaunchKernelGGL(mq_roots_kernel, dim3((nroots + kMqBlock - 1) / kMqBlock), dim3(kMqBlock), 0, s,
+                     a, root_v, root_q, nroots);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_mq_round(const MqArgs& a, bool all, int grid, hipStream_t s) {
+  // a kernel clears the histograms (the round may be captured in a graph)
+  launch_zero_u32(a.hist, static_cast<uint64_t>(a.nq_count) * kMqHistStride, s);
+  launch_zero_active(a.ctl, s);
+  hipLaunchKernelGGL(mq_hist_kernel, dim3(grid), dim3(kMqBlock), 0, s, a);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(mq_select_batch_kernel, dim3(a.nq_count), dim3(kMqSelBlock), 0, s, a, all ? 1 : 0);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(mq_make_batch_kernel, dim3(grid), dim3(kMqBlock), 0, s, a);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((mq_walk_kernel<false>), dim3(grid), dim3(kMqBlock), 0, s, a);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(mq_select_bound_kernel, dim3(a.nq_count), dim3(kMqSelBlock), 0, s, a);
+  CPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((mq_walk_kernel<true>), dim3(grid), dim3(kMqBlock), 0, s, a);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_mq_build_keys(const MqArgs& a, uint64_t nmax, uint64_t* key, uint32_t* val,
+                          unsigned long long* count, unsigned long long* per_q, int grid,
+                          hipStream_t s) {
+  if (nmax == 0) return;
+  hipLaunchKernelGGL(mq_build_keys_kernel, dim3(grid), dim3(kMqBlock), 0, s, a, key, val, count, per_q);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_mq_gather_costs(const uint32_t* val, const float* cost, uint64_t n, float* out,
+                            hipStream_t s) {
+  if (n == 0) return;
+  const uint64_t g = std::min<uint64_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(mq_gather_costs_kernel, dim3(static_cast<unsigned>(g)), dim3(256), 0, s, val,
+                     cost, n, out);
+  CPG_HIP(hipGetLastError());
+}
+
+}  // namespace cpg

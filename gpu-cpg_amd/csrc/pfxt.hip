@@ -31,6 +31,7 @@
 // decides emission.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "walk.hpp"
 
 namespace cpg {
 
@@ -341,66 +342,22 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < nb;
        base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t i = base + threadIdx.x;
-    int mode = 2;  // 0: at a chain vertex, 1: scanning its deviations, 2: done
-    int p = 0, v = -1, lvl = 0;
-    float c = 0.0f, X = __builtin_huge_valf();
+    int p = 0, lvl = 0;
+    float c = 0.0f;
+    int to = -1;
     if (i < nb) {
       p = static_cast<int>(a.batch[i]);
       c = a.nodes.cost[p];
-      v = a.nodes.to[p];
+      to = a.nodes.to[p];
       lvl = a.nodes.level[p];
-      if (bounded) X = (U - c) + ldexpf(fabsf(U) + fabsf(c) + mag, -17);
-      mode = v != -1 ? 0 : 2;
     }
-    int j = 0, je = 0, sv = -1;
-    float dv = 0.0f;
-    int steps = 0;
-    while (__ballot(mode != 2)) {
-      bool emit = false;
+    ChainWalk cw_state;
+    cw_state.start(i < nb, c, to, U, bounded, mag);
+    while (__ballot(cw_state.mode != 2)) {
       float cc = 0.0f;
-      int cw = 0;
-      // a walk visits each chain vertex and each of its deviations at most
-      // once; more steps than that means a corrupt index, so stop and flag
-      if (mode != 2 && ++steps > a.step_limit) {
-        atomicExch(&ctl->err, 1);
-        mode = 2;
-      }
-      int from = v;
-      if (mode == 0) {
-        const int4 r = a.vrec[v];
-        ++hops;
-        if (__int_as_float(r.x) > X) {
-          v = r.y;
-          if (v == -1) mode = 2;
-        } else {
-          const int2 di = a.devinfo[v];
-          j = di.x;
-          je = di.x + di.y;
-          sv = r.z;
-          dv = __int_as_float(r.w);
-          mode = 1;
-        }
-      } else if (mode == 1) {
-        bool leave = j >= je;
-        if (!leave) {
-          const int4 d = a.devs[j];
-          ++devv;
-          if (__int_as_float(d.w) > X) {
-            leave = true;
-          } else {
-            cc = ((c + __int_as_float(d.z)) + __int_as_float(d.y)) - dv;
-            ++j;
-            if (a.strict ? cc < U : cc <= U) {
-              emit = true;
-              cw = d.x;
-            }
-          }
-        }
-        if (leave) {
-          v = sv;
-          mode = v == -1 ? 2 : 0;
-        }
-      }
+      int cw = 0, from = 0;
+      const bool emit = cw_state.step(a.vrec, a.devinfo, a.devs, U, a.strict != 0, a.step_limit,
+                                      &ctl->err, &hops, &devv, &cc, &cw, &from);
       const uint64_t m = __ballot(emit);
       if (m) {
         const int add = static_cast<int>(__popcll(m));

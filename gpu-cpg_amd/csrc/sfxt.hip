@@ -39,7 +39,12 @@ namespace {
 
 constexpr int kSfxtBlock = 256;
 constexpr int kLdsQueue = 2048;
-constexpr unsigned kSpinLimit = 1u << 22;
+// Polls of a grid-barrier wait before the level loop reports an error; a
+// build that runs blocks on CPU threads (the fiber emulator) raises it.
+#ifndef CPG_SPIN_LIMIT
+#define CPG_SPIN_LIMIT (1u << 22)
+#endif
+constexpr unsigned kSpinLimit = CPG_SPIN_LIMIT;
 constexpr int kThreadSort = 16;    // deviation lists up to this length: one thread sorts
 constexpr int kWaveSort = 64;      // up to this length: one wave ranks them in registers
 constexpr int kLdsSort = kSfxtLdsSort;  // up to this length: one block sorts in LDS

@@ -153,6 +153,26 @@ int64_t cpg_bvss_row_sources(cpg_handle* h, int row, int* out, int64_t cap);
 int64_t cpg_bvss_discover(cpg_handle* h, int nfront, const int* fptr, const int* fsrc, int mode,
                           int dense_min, int* out_f, int* out_u, int* out_v, int64_t cap);
 
+/* ---- batched designs ----
+ * Many independent designs, one K-path query each, solved together: the
+ * designs' disjoint union gets one SFXT and one multi-query PFXT whose
+ * rounds run for every query at once (after the first round they replay
+ * a captured HIP graph).  Each query's result equals cpg_query's on its
+ * design alone (same options). */
+typedef struct cpg_batch cpg_batch;
+cpg_batch* cpg_batch_create(void);
+void cpg_batch_destroy(cpg_batch* b);
+/* Adds a design (edge list, same semantics as cpg_load_edges) and its K;
+ * returns the query index. */
+int cpg_batch_add_edges(cpg_batch* b, int n, int64_t m, const int* eu, const int* ev,
+                        const float* ew, int64_t k);
+int cpg_batch_num_queries(cpg_batch* b);
+int cpg_batch_run(cpg_batch* b, const cpg_query_opts* o);
+/* First k costs of query q (ascending, exact), as cpg_get_costs. */
+int64_t cpg_batch_get_costs(cpg_batch* b, int q, int64_t k, float* out);
+/* Whole-batch timings (sfxt/pfxt/select/query ms, rounds, generated). */
+int cpg_batch_timings(cpg_batch* b, cpg_timing* out);
+
 /* ---- static cache, gpucpg/gpucpg.cuh:186-189 ---- */
 int cpg_cache_enable(cpg_handle* h, int enabled);
 int cpg_cache_clear(cpg_handle* h);

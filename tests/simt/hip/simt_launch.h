@@ -4,10 +4,18 @@
 
 namespace simt {
 void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, bool cooperative);
+// Stream capture: while a capture is open, launches are recorded (with
+// their arguments) instead of run; a graph launch replays them in order.
+bool capturing();
+void record(std::function<void()> launch_fn);
 
 template <class F, class... A>
 void launch(F kernel, dim3 grid, dim3 block, bool cooperative, A... args) {
   std::function<void()> body = [=]() { kernel(args...); };
+  if (capturing()) {
+    record([=]() { run_grid(body, grid, block, cooperative); });
+    return;
+  }
   run_grid(body, grid, block, cooperative);
 }
 }  // namespace simt

@@ -10,6 +10,7 @@
 // that grid-wide barriers made of atomics can complete.
 #include "hip/hip_runtime.h"
 
+#include <sys/mman.h>
 #include <ucontext.h>
 
 #include <atomic>
@@ -27,9 +28,29 @@ enum State { kRun, kBar, kWave, kDone };
 constexpr size_t kStack = 256 * 1024;
 
 struct Block;
+// A fiber stack mapped with mmap rather than taken from the heap, so that
+// AddressSanitizer builds never see stack frames inside heap chunks.
+struct Stack {
+  char* p = nullptr;
+  Stack() = default;
+  Stack(const Stack&) = delete;
+  Stack& operator=(const Stack&) = delete;
+  Stack(Stack&& o) noexcept : p(o.p) { o.p = nullptr; }
+  ~Stack() {
+    if (p) munmap(p, kStack);
+  }
+  char* get() {
+    if (!p) {
+      void* m = mmap(nullptr, kStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+      if (m == MAP_FAILED) std::abort();
+      p = static_cast<char*>(m);
+    }
+    return p;
+  }
+};
 struct Lane {
   ucontext_t ctx;
-  std::vector<char> stack;
+  Stack stack;
   int tid = 0;
   State st = kRun;
   Block* blk = nullptr;
@@ -90,9 +111,8 @@ void run_block(Block& B) {
     L.tid = t;
     L.st = kRun;
     L.blk = &B;
-    if (L.stack.size() != kStack) L.stack.assign(kStack, 0);
     getcontext(&L.ctx);
-    L.ctx.uc_stack.ss_sp = L.stack.data();
+    L.ctx.uc_stack.ss_sp = L.stack.get();
     L.ctx.uc_stack.ss_size = kStack;
     L.ctx.uc_link = nullptr;
     const uintptr_t p = reinterpret_cast<uintptr_t>(&L);
@@ -219,8 +239,14 @@ struct simt_stream { int dummy; };
 const char* hipGetErrorString(hipError_t) { return "simt"; }
 hipError_t hipGetLastError() { return hipSuccess; }
 hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+// SIMT_CUS sets the reported compute-unit count (default 2), so that
+// grids sized from it match a real device's.
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
-  *v = a == hipDeviceAttributeMultiprocessorCount ? 2 : 1;
+  static const int cus = [] {
+    const char* e = std::getenv("SIMT_CUS");
+    return e ? std::max(1, std::atoi(e)) : 2;
+  }();
+  *v = a == hipDeviceAttributeMultiprocessorCount ? cus : 1;
   return hipSuccess;
 }
 hipError_t simt_malloc(void** p, size_t bytes) {
@@ -245,3 +271,36 @@ hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
   return hipSuccess;
 }
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+
+struct simt_graph {
+  std::vector<std::function<void()>> launches;
+  int refs = 1;
+};
+namespace {
+thread_local simt_graph* t_capture = nullptr;
+}
+bool simt::capturing() { return t_capture != nullptr; }
+void simt::record(std::function<void()> launch_fn) { t_capture->launches.push_back(std::move(launch_fn)); }
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  t_capture = new simt_graph();
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+  *g = t_capture;
+  t_capture = nullptr;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, void*, size_t) {
+  g->refs++;
+  *e = g;
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto& f : e->launches) f();
+  return hipSuccess;
+}
+static void graph_release(simt_graph* g) {
+  if (g && --g->refs == 0) delete g;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { graph_release(e); return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t g) { graph_release(g); return hipSuccess; }
