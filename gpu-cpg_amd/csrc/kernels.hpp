@@ -17,7 +17,7 @@ struct SfxtCtl {
   unsigned levelized;           // vertices peeled
   unsigned max_abs_distf_bits;  // max |distf| as float bits
   int nsv_err;                  // 1: nsv descent found no answer, 2: long list without devs_tmp
-  unsigned pad;
+  unsigned bar_tail;            // qtail when the last grid barrier completed
   unsigned long long nbig;      // deviation lists one wave sorts (front of big[])
   unsigned long long nbig_l;    // deviation lists one block sorts (back of big[])
 };

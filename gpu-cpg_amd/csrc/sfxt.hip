@@ -60,8 +60,11 @@ __device__ inline unsigned atomic_load_rlx(unsigned* p) {
 // the wait polls with relaxed loads so it does not invalidate caches on
 // every probe.  A spin that exceeds kSpinLimit raises ctl->err and every
 // later barrier returns false at once, so the kernel always drains.
-// Returns false on error; *qtail receives the queue tail read after the
-// barrier by one thread and broadcast through LDS.
+// Returns false on error; *qtail receives the queue tail as it stood when
+// the barrier completed: the last block to arrive (every block's pushes
+// are done) stores it in bar_tail before releasing the others.  Reading
+// qtail itself after leaving would race with blocks already pushing the
+// next level, and blocks could then disagree on the level's end.
 __device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned* qtail) {
   __syncthreads();
   __shared__ int s_ok;
@@ -73,6 +76,9 @@ __device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned* qtail) {
     const unsigned arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == nblocks - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(&ctl->bar_tail, atomic_load_rlx(&ctl->qtail), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -85,7 +91,7 @@ __device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned* qtail) {
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
-    s_tail = atomic_load_rlx(&ctl->qtail);
+    s_tail = atomic_load_rlx(&ctl->bar_tail);
     s_ok = ok;
   }
   __syncthreads();

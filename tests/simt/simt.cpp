@@ -15,6 +15,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <map>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -28,8 +29,11 @@ enum State { kRun, kBar, kWave, kDone };
 constexpr size_t kStack = 256 * 1024;
 
 struct Block;
-// A fiber stack mapped with mmap rather than taken from the heap, so that
-// AddressSanitizer builds never see stack frames inside heap chunks.
+// Fiber stacks are mapped with mmap rather than taken from the heap (so
+// that AddressSanitizer builds never see stack frames inside heap chunks)
+// and kept in a process-wide pool for reuse by later blocks and launches.
+std::mutex g_stack_mu;
+std::vector<char*> g_stack_pool;
 struct Stack {
   char* p = nullptr;
   Stack() = default;
@@ -37,14 +41,23 @@ struct Stack {
   Stack& operator=(const Stack&) = delete;
   Stack(Stack&& o) noexcept : p(o.p) { o.p = nullptr; }
   ~Stack() {
-    if (p) munmap(p, kStack);
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_stack_mu);
+    g_stack_pool.push_back(p);
   }
   char* get() {
-    if (!p) {
-      void* m = mmap(nullptr, kStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
-      if (m == MAP_FAILED) std::abort();
-      p = static_cast<char*>(m);
+    if (p) return p;
+    {
+      std::lock_guard<std::mutex> lk(g_stack_mu);
+      if (!g_stack_pool.empty()) {
+        p = g_stack_pool.back();
+        g_stack_pool.pop_back();
+        return p;
+      }
     }
+    void* m = mmap(nullptr, kStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (m == MAP_FAILED) std::abort();
+    p = static_cast<char*>(m);
     return p;
   }
 };
@@ -188,7 +201,9 @@ void block_sync() {
   park(kBar);
 }
 
-void relax() { std::this_thread::yield(); }
+// A polling wait (a grid barrier) sleeps briefly so that the blocks it
+// waits for get the CPU: blocks are OS threads, often more than cores.
+void relax() { std::this_thread::sleep_for(std::chrono::microseconds(20)); }
 
 void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, bool cooperative) {
   const unsigned nb = grid.x * grid.y * grid.z;
@@ -249,11 +264,49 @@ hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
   *v = a == hipDeviceAttributeMultiprocessorCount ? cus : 1;
   return hipSuccess;
 }
-hipError_t simt_malloc(void** p, size_t bytes) {
-  *p = std::calloc(1, bytes ? bytes : 1);
-  return *p ? hipSuccess : 2;
+// SIMT_GUARD=1 places every device buffer at the end of its own mapping,
+// against an inaccessible page, so a read or write past the end of a
+// buffer (by more than the 16-byte alignment slack) faults at once.
+namespace {
+bool guard_on() {
+  static const bool on = [] { const char* e = std::getenv("SIMT_GUARD"); return e && *e == '1'; }();
+  return on;
 }
-hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+std::mutex g_map_mu;
+std::map<void*, std::pair<void*, size_t>> g_maps;  // buffer -> (mapping, length)
+}  // namespace
+hipError_t simt_malloc(void** p, size_t bytes) {
+  if (!bytes) bytes = 1;
+  if (!guard_on()) {
+    *p = std::calloc(1, bytes);
+    return *p ? hipSuccess : 2;
+  }
+  const size_t pg = 4096;
+  const size_t body = (bytes + 15) / 16 * 16;
+  const size_t len = (body + pg - 1) / pg * pg + pg;
+  void* m = mmap(nullptr, len, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+  if (m == MAP_FAILED) return 2;
+  char* guard = static_cast<char*>(m) + len - pg;
+  mprotect(guard, pg, PROT_NONE);
+  void* b = guard - body;
+  std::lock_guard<std::mutex> lk(g_map_mu);
+  g_maps[b] = {m, len};
+  *p = b;
+  return hipSuccess;
+}
+hipError_t hipFree(void* p) {
+  if (!p) return hipSuccess;
+  if (!guard_on()) {
+    std::free(p);
+    return hipSuccess;
+  }
+  std::lock_guard<std::mutex> lk(g_map_mu);
+  auto it = g_maps.find(p);
+  if (it == g_maps.end()) std::abort();
+  munmap(it->second.first, it->second.second);
+  g_maps.erase(it);
+  return hipSuccess;
+}
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
   std::memcpy(d, s, n);
   return hipSuccess;
