@@ -47,6 +47,16 @@ class Timing(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "submitted", "admitted", "completed", "rounds", "compactions", "restarts",
+        "peak_nodes", "node_cap", "peak_index_bytes", "in_flight", "queued")] + [
+        (n, ctypes.c_double) for n in ("sfxt_ms", "round_ms", "extract_ms", "compact_ms")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class EdgeList(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("m", ctypes.c_int64),
                 ("u", ctypes.POINTER(ctypes.c_int)), ("v", ctypes.POINTER(ctypes.c_int)),
@@ -86,6 +96,19 @@ ABI = {
     "cpg_batch_run": (ctypes.c_int, [_H, ctypes.POINTER(QueryOpts)]),
     "cpg_batch_get_costs": (_I64, [_H, ctypes.c_int, _I64, _PF]),
     "cpg_batch_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
+    "cpg_server_create": (_H, [_I64, ctypes.c_int, _I64]),
+    "cpg_server_destroy": (None, [_H]),
+    "cpg_server_add_graph": (ctypes.c_int, [_H, ctypes.c_int, _I64, _PI, _PI, _PF]),
+    "cpg_server_submit": (_I64, [_H, ctypes.c_int, _I64, ctypes.c_void_p]),
+    "cpg_server_step": (_I64, [_H]),
+    "cpg_server_poll": (_I64, [_H, ctypes.POINTER(_I64), _I64]),
+    "cpg_server_get_costs": (_I64, [_H, _I64, _I64, _PF]),
+    "cpg_server_query_times": (ctypes.c_int, [_H, _I64, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double)]),
+    "cpg_server_release": (ctypes.c_int, [_H, _I64]),
+    "cpg_server_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(ServerStats)]),
+    "cpg_now_ms": (ctypes.c_double, []),
     "cpg_bvss_build": (ctypes.c_int, [_H]),
     "cpg_bvss_info": (ctypes.c_int, [_H, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                      ctypes.POINTER(_I64)]),
@@ -237,11 +260,13 @@ class CpGen:
 
     def report_paths(self, k, *args, root_tie_quirk=True, batch=0, max_nodes=0,
                      collect_stats=False, init_split_perc=0.005, kernel_timing=False,
-                     sfxt_blocks=0, bound_probe=0, **kwargs):
+                     sfxt_blocks=0, bound_probe=0, endpoint_mask=None, **kwargs):
         """Runs the GPG query for the k lowest-cost paths.  Positional
         arguments after k (max_dev_lvls, enable_compress, methods, ...) are
         accepted for call compatibility with the reference; the engine always
-        runs its exact GPU path."""
+        runs its exact GPU path.  endpoint_mask: optional array of n bytes
+        (nonzero = chosen sink) restricting the paths to those ending at the
+        chosen sinks."""
         o = QueryOpts()
         self.L.cpg_query_opts_default(ctypes.byref(o))
         o.root_tie_quirk = 1 if root_tie_quirk else 0
@@ -252,6 +277,12 @@ class CpGen:
         o.kernel_timing = 1 if kernel_timing else 0
         o.sfxt_blocks = int(sfxt_blocks)
         o.bound_probe = int(bound_probe)
+        mask = None
+        if endpoint_mask is not None:
+            mask = np.ascontiguousarray(endpoint_mask, dtype=np.uint8)
+            if mask.shape != (self.num_verts(),):
+                raise ValueError("endpoint_mask must hold one byte per vertex")
+            o.endpoint_mask = mask.ctypes.data
         _c(self, self.L.cpg_query(self.h, int(k), ctypes.byref(o)))
         self._k = int(k)
 
@@ -397,3 +428,72 @@ class Batch:
         t = Timing()
         _check(self.L.cpg_batch_timings(self.h, ctypes.byref(t)), self.L)
         return t.as_dict()
+
+
+class Server:
+    """Resident graphs and a stream of top-K queries with endpoint subsets,
+    their PFXT rounds batched continuously (cpg_server_*)."""
+
+    def __init__(self, budget_bytes=0, max_in_flight=0, initial_nodes=0, library=None):
+        self.L = library or lib()
+        self.h = self.L.cpg_server_create(int(budget_bytes), int(max_in_flight), int(initial_nodes))
+        if not self.h:
+            raise CpgError(self.L.cpg_last_error().decode())
+        self._n = []
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.cpg_server_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def add_graph(self, n, u, v, w):
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        w = np.ascontiguousarray(w, np.float32)
+        gid = _check(self.L.cpg_server_add_graph(self.h, int(n), len(u), _iptr(u), _iptr(v),
+                                                 _fptr(w)), self.L)
+        self._n.append(int(n))
+        return gid
+
+    def submit(self, graph, k, endpoint_mask=None):
+        ptr = None
+        if endpoint_mask is not None:
+            m = np.ascontiguousarray(endpoint_mask, dtype=np.uint8)
+            if m.shape != (self._n[graph],):
+                raise ValueError("endpoint_mask must hold one byte per vertex")
+            ptr = m.ctypes.data
+        return _check(self.L.cpg_server_submit(self.h, int(graph), int(k), ptr), self.L)
+
+    def step(self):
+        return _check(self.L.cpg_server_step(self.h), self.L)
+
+    def poll(self, cap=4096):
+        ids = (ctypes.c_int64 * cap)()
+        n = _check(self.L.cpg_server_poll(self.h, ids, cap), self.L)
+        return [ids[i] for i in range(n)]
+
+    def get_slacks(self, qid, k):
+        out = np.zeros(max(int(k), 1), np.float32)
+        n = _check(self.L.cpg_server_get_costs(self.h, int(qid), int(k), _fptr(out)), self.L)
+        return out[:n]
+
+    def times(self, qid):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self.L.cpg_server_query_times(self.h, int(qid), ctypes.byref(a), ctypes.byref(b),
+                                             ctypes.byref(c)), self.L)
+        return a.value, b.value, c.value
+
+    def release(self, qid):
+        _check(self.L.cpg_server_release(self.h, int(qid)), self.L)
+
+    def stats(self):
+        st = ServerStats()
+        _check(self.L.cpg_server_get_stats(self.h, ctypes.byref(st)), self.L)
+        return st.as_dict()
+
+
+def now_ms():
+    return lib().cpg_now_ms()

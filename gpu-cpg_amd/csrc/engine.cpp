@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -58,6 +60,13 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
 };
 
@@ -68,6 +77,45 @@ void h2d(T* d, const T* h, size_t n, hipStream_t s) {
 template <class T>
 void d2h(T* h, const T* d, size_t n, hipStream_t s) {
   if (n) CPG_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+}
+
+// Roots of a query: the sources (ascending id) with a finite distance
+// (gpucpg/gpucpg.cu:18502-18513).  With the tie quirk, the initial split of
+// gpucpg/gpucpg.cu:19136-19180: the short pile keeps the first (int)(p*n+1)
+// roots with cost <= split = sorted[(size_t)(p*n)]; roots tied at <= split
+// beyond that count are lost.  sd holds the sources' int distances; kp
+// receives one keep flag per source, minc the smallest kept cost.
+void pick_roots(const std::vector<int>& sd, bool quirk, float perc, std::vector<uint8_t>& kp,
+                int64_t& nroots, int64_t& dropped, float& minc) {
+  const int nsrc = static_cast<int>(sd.size());
+  kp.assign(std::max(1, nsrc), 0);
+  std::vector<float> rc;
+  rc.reserve(nsrc);
+  for (int i = 0; i < nsrc; ++i)
+    if (sd[i] != kIntMax) {
+      kp[i] = 1;
+      rc.push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+    }
+  nroots = static_cast<int64_t>(rc.size());
+  dropped = 0;
+  if (quirk && nroots > 0) {
+    std::vector<float> sorted = rc;
+    std::sort(sorted.begin(), sorted.end());
+    const float pn = perc * static_cast<float>(nroots);
+    const float split = sorted[static_cast<size_t>(pn)];
+    const int num_short = static_cast<int>(pn + 1);
+    int kept_le = 0;
+    for (int i = 0; i < nsrc; ++i) {
+      if (!kp[i]) continue;
+      const float c = static_cast<float>(sd[i]) / static_cast<float>(kScaleUp);
+      if (c > split) continue;
+      if (kept_le < num_short) ++kept_le;
+      else { kp[i] = 0; ++dropped; }
+    }
+  }
+  minc = __builtin_huge_valf();
+  for (int i = 0; i < nsrc; ++i)
+    if (kp[i]) minc = std::min(minc, static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
 }
 
 // A pair of HIP events bracketing work on one stream.
@@ -110,6 +158,8 @@ struct cpg_handle {
   DevBuf<int2> devinfo, lift;
   DevBuf<SfxtCtl> sctl;
   bool sfxt_valid = false;
+  std::vector<uint8_t> sfxt_sinks;  // endpoint subset of the current SFXT, per sink (empty = all)
+  DevBuf<uint8_t> emask;
   int diameter = 0;
   int sfxt_blocks = 0;       // level-loop grid requested (0 = automatic)
   int last_diameter = 0;     // levels of the last SFXT on this graph (automatic grid)
@@ -240,13 +290,32 @@ struct cpg_handle {
     return static_cast<int>(std::min<int64_t>(64, std::max<int64_t>(16, (per_level + 255) / 256)));
   }
 
-  // Runs the SFXT unless a cached one is valid.  Returns elapsed ms.
-  double run_sfxt(bool count_cache) {
+  // The endpoint subset of an n-byte mask as one flag per sink (in sink
+  // order); empty when the mask is null or chooses every sink.
+  std::vector<uint8_t> sink_subset(const uint8_t* mask) const {
+    std::vector<uint8_t> f;
+    if (!mask) return f;
+    bool all = true;
+    f.resize(g.sinks.size());
+    for (size_t i = 0; i < g.sinks.size(); ++i) {
+      f[i] = mask[g.sinks[i]] ? 1 : 0;
+      all = all && f[i];
+    }
+    if (all) f.clear();
+    return f;
+  }
+
+  // Runs the SFXT for the endpoint subset of mask (n bytes, null = every
+  // sink) unless a cached SFXT for that subset is valid.  Returns elapsed ms.
+  double run_sfxt(bool count_cache, const uint8_t* mask = nullptr) {
     upload_graph();
-    if (sfxt_valid && cache_enabled) {
+    std::vector<uint8_t> subset = sink_subset(mask);
+    if (sfxt_valid && cache_enabled && subset == sfxt_sinks) {
       if (count_cache) ++cache_hits;
       return 0.0;
     }
+    hb.valid = false;
+    sfxt_valid = false;
     if (cache_enabled && count_cache) ++cache_misses;
     const double t0 = now_ms();
     const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
@@ -267,6 +336,11 @@ struct cpg_handle {
       a.vrec = vrec.p; a.devinfo = devinfo.p; a.devs = devs.p; a.big = big.p;
       a.devs_tmp = g.max_out_degree > kSfxtLdsSort ? devs_tmp.p : nullptr;
       a.ctl = sctl.p;
+      if (!subset.empty()) {
+        emask.reserve(n);
+        h2d(emask.p, mask, n, stream);
+        a.sink_mask = emask.p;
+      }
       if (kernel_timing) {
         ev_sfxt.create();
         ev_levels.create();
@@ -299,6 +373,7 @@ struct cpg_handle {
     uint32_t bits = c.max_abs_distf_bits;
     std::memcpy(&max_abs_distf, &bits, 4);
     sfxt_valid = true;
+    sfxt_sinks = std::move(subset);
     h_succ.clear();
     return now_ms() - t0;
   }
@@ -315,6 +390,12 @@ struct cpg_handle {
     hist_part.reserve(3 * static_cast<size_t>(kHistParts) * kHistBins);
     pctl.reserve(1);
     node_cap = cap;
+  }
+
+  // Bound on one chain walk's steps: chain vertices plus their deviations.
+  int step_limit() const {
+    return static_cast<int>(std::min<int64_t>(
+        0x3fffffff, (static_cast<int64_t>(diameter) + 2) * (g.max_out_degree + 2) + 16));
   }
 
   NodeStore node_store() {
@@ -417,7 +498,7 @@ struct cpg_handle {
     if (o.sfxt_blocks < 0) throw std::runtime_error("sfxt_blocks must be >= 0");
     sfxt_blocks = o.sfxt_blocks;
     const double tq0 = now_ms();
-    timing.sfxt_ms = run_sfxt(true);
+    timing.sfxt_ms = run_sfxt(true, o.endpoint_mask);
     if (k <= 0 || g.n == 0) {
       timing.query_ms = now_ms() - tq0;
       have_result = true;
@@ -431,36 +512,10 @@ struct cpg_handle {
     launch_gather_int(dist.p, srcs.p, nsrc, src_dist.p, stream);
     d2h(sd.data(), src_dist.p, nsrc, stream);
     CPG_HIP(hipStreamSynchronize(stream));
-    std::vector<uint8_t> kp(std::max(1, nsrc), 0);
-    std::vector<float> rc;
-    rc.reserve(nsrc);
-    for (int i = 0; i < nsrc; ++i)
-      if (sd[i] != kIntMax) {
-        kp[i] = 1;
-        rc.push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
-      }
-    const int64_t nroots = static_cast<int64_t>(rc.size());
-    int64_t dropped = 0;
-    if (o.root_tie_quirk && nroots > 0) {
-      // initial split, gpucpg/gpucpg.cu:19136-19180: the short pile keeps
-      // the first (int)(p*n+1) roots (ascending id) with cost <= split
-      std::vector<float> sorted = rc;
-      std::sort(sorted.begin(), sorted.end());
-      const float pn = o.init_split_perc * static_cast<float>(nroots);
-      const float split = sorted[static_cast<size_t>(pn)];
-      const int num_short = static_cast<int>(pn + 1);
-      int kept_le = 0;
-      for (int i = 0; i < nsrc; ++i) {
-        if (!kp[i]) continue;
-        const float c = static_cast<float>(sd[i]) / static_cast<float>(kScaleUp);
-        if (c > split) continue;
-        if (kept_le < num_short) ++kept_le;
-        else { kp[i] = 0; ++dropped; }
-      }
-    }
-    float minc = __builtin_huge_valf();
-    for (int i = 0; i < nsrc; ++i)
-      if (kp[i]) minc = std::min(minc, static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+    std::vector<uint8_t> kp;
+    int64_t nroots = 0, dropped = 0;
+    float minc = 0.0f;
+    pick_roots(sd, o.root_tie_quirk != 0, o.init_split_perc, kp, nroots, dropped, minc);
     timing.roots = nroots - dropped;
     timing.roots_dropped = dropped;
     keep.reserve(std::max(1, nsrc));
@@ -495,8 +550,7 @@ struct cpg_handle {
       wa.nodes = ns; wa.ckeys = ckeys.p; wa.ckeys_cap = cap; wa.ctl = pctl.p;
       wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = g.max_abs_wgt;
       wa.collect_stats = o.collect_stats;
-      wa.step_limit = static_cast<int>(std::min<int64_t>(
-          0x3fffffff, (static_cast<int64_t>(diameter) + 2) * (g.max_out_degree + 2) + 16));
+      wa.step_limit = step_limit();
       const int64_t nkept = nroots - dropped;
       const bool probe = nkept > 0 && (o.bound_probe > 0 ||
                                        (o.bound_probe == 0 && nkept * (diameter + 1) > (1 << 20)));
@@ -889,6 +943,11 @@ struct cpg_batch {
       m.k = static_cast<unsigned long long>(std::max<int64_t>(0, designs[q].k));
       m.batch_target = std::max<unsigned long long>(256, m.k / 8);
       m.done = (cnt == 0 || m.k == 0) ? 1 : 0;
+      m.vrec = uh.vrec.p;
+      m.devinfo = uh.devinfo.p;
+      m.devs = uh.devs.p;
+      m.mag = 2.0f * uh.max_abs_distf + uh.g.max_abs_wgt + 1.0f;
+      m.step_limit = uh.step_limit();
       need += 16ull * m.k + static_cast<uint64_t>(cnt) + 4096;
       timing.roots += cnt;
       timing.roots_dropped += static_cast<int64_t>(nr) - cnt;
@@ -953,7 +1012,7 @@ struct cpg_batch {
     CPG_HIP(hipMemsetAsync(nsel.p, 0, sizeof(unsigned long long), stream));
     CPG_HIP(hipMemsetAsync(per_q.p, 0, sizeof(unsigned long long) * nqc, stream));
     MqArgs a = args(nqc);
-    launch_mq_build_keys(a, nn, skey[0].p, sval[0].p, nsel.p, per_q.p, uh.grid, stream);
+    launch_mq_build_keys(a, nullptr, nn, skey[0].p, sval[0].p, nsel.p, per_q.p, uh.grid, stream);
     unsigned long long nl = 0;
     std::vector<unsigned long long> pq(nqc);
     d2h(&nl, nsel.p, 1, stream);
@@ -986,12 +1045,417 @@ struct cpg_batch {
     MqArgs a{};
     a.nodes = NodeStore{n_cost.p, n_to.p, n_from.p, n_parent.p, n_level.p, n_key.p, n_flag.p, cap};
     a.nq = nq.p;
-    a.vrec = uh.vrec.p; a.devinfo = uh.devinfo.p; a.devs = uh.devs.p; a.distf = uh.distf.p;
+    a.distf = uh.distf.p;
     a.batch = batch.p; a.hist = hist.p; a.qs = qs.p; a.ctl = mctl.p; a.nq_count = nqc;
-    a.max_abs_distf = uh.max_abs_distf; a.max_abs_wgt = uh.g.max_abs_wgt;
-    a.step_limit = static_cast<int>(std::min<int64_t>(
-        0x3fffffff, (static_cast<int64_t>(uh.diameter) + 2) * (uh.g.max_out_degree + 2) + 16));
     return a;
+  }
+};
+
+// Served queries: resident graphs, a stream of top-K queries each with its
+// own endpoint subset, continuous batching of their PFXT rounds.
+//
+// A query is admitted between rounds when a slot is free and the device
+// bytes of the in-flight queries' deviation indexes stay within the
+// budget.  Admission runs the query's SFXT for its endpoint subset on its
+// graph's handle and takes the index buffers over (a buffer swap, no
+// copy), then appends its roots to the shared node store; every round then
+// expands every in-flight query at once (the multi-query kernels of
+// mq.hip, each query reading its own index).  A query whose expansion is
+// done is extracted at the end of that round (its paths sorted by cost),
+// and its slot and index are released.  Nodes of finished queries stay in
+// the store until a compaction copies the live ones to the second buffer
+// set; a round that overflows the store restarts the in-flight queries
+// from their roots in a store twice as large.
+// Node-store headroom reserved per requested path at admission (a test
+// build sets 0 to force the overflow restart).
+#ifndef CPG_SERVER_NODES_PER_PATH
+#define CPG_SERVER_NODES_PER_PATH 16ull
+#endif
+
+struct cpg_server {
+  struct IndexSet {
+    DevBuf<int4> vrec, devs;
+    DevBuf<int2> devinfo;
+    DevBuf<float> distf;
+  };
+  struct Graph {
+    std::unique_ptr<cpg_handle> h;
+    std::vector<std::unique_ptr<IndexSet>> pool;  // index buffers of finished queries
+    int64_t index_bytes = 0;
+  };
+  struct Query {
+    int64_t id = 0;
+    int graph = 0;
+    int64_t k = 0;
+    std::vector<uint8_t> mask;  // n bytes, or empty for every sink
+    double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
+    int slot = -1;
+    int state = 0;              // 0 queued, 1 in flight, 2 done
+    std::unique_ptr<IndexSet> index;
+    std::vector<int> roots;     // kept roots (vertex ids, ascending)
+    MqQuery init{};             // slot state at admission
+    std::vector<float> result;
+  };
+  struct Store {
+    DevBuf<float> cost;
+    DevBuf<int> to, from, parent, level, nq;
+    DevBuf<uint32_t> key;
+    DevBuf<uint8_t> flag;
+    uint64_t cap = 0;
+    void reserve(uint64_t c) {
+      cost.reserve(c); to.reserve(c); from.reserve(c); parent.reserve(c); level.reserve(c);
+      nq.reserve(c); key.reserve(c + 16); flag.reserve(c + 16);
+      cap = c;
+    }
+    NodeStore view() { return NodeStore{cost.p, to.p, from.p, parent.p, level.p, key.p, flag.p, cap}; }
+  };
+
+  std::vector<Graph> graphs;
+  std::deque<int64_t> queue;  // submitted, not yet admitted
+  std::unordered_map<int64_t, std::unique_ptr<Query>> queries;
+  std::vector<int64_t> slot_query;  // query id per slot (-1 free)
+  std::vector<int> free_slots, dead_slots;
+  std::vector<int64_t> completed;   // finished since the last poll
+  int64_t next_id = 1;
+  int64_t budget = 0;
+  int64_t index_bytes = 0;          // indexes of the in-flight queries
+  int nslots = 0;
+  int in_flight = 0;
+  hipStream_t stream = nullptr;
+  int grid = 0;
+  Store st[2];
+  int cur = 0;
+  uint64_t node_n = 0;
+  std::vector<MqQuery> hq;
+  DevBuf<uint32_t> batch;
+  DevBuf<unsigned> hist, tiles, ghist;
+  DevBuf<MqQuery> qs;
+  DevBuf<MqCtl> mctl;
+  DevBuf<int> root_v, root_q, newidx;
+  DevBuf<uint8_t> sel;
+  DevBuf<uint64_t> skey[2];
+  DevBuf<uint32_t> sval[2];
+  DevBuf<unsigned long long> nsel, per_q;
+  DevBuf<float> out_cost;
+  cpg_server_stats stats{};
+  bool ready = false;
+
+  uint64_t first_cap = uint64_t(1) << 24;
+
+  cpg_server(int64_t budget_bytes, int max_in_flight, int64_t initial_nodes)
+      : budget(budget_bytes > 0 ? budget_bytes : (int64_t(64) << 30)),
+        nslots(max_in_flight > 0 ? std::min(max_in_flight, 256) : 64) {
+    if (initial_nodes > 0) first_cap = std::min<uint64_t>(static_cast<uint64_t>(initial_nodes), 0x7fffffffull);
+  }
+  ~cpg_server() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void init() {
+    if (ready) return;
+    CPG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    int dev = 0, cus = 0;
+    CPG_HIP(hipGetDevice(&dev));
+    CPG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = std::max(1, cus) * 8;
+    slot_query.assign(nslots, -1);
+    for (int i = nslots - 1; i >= 0; --i) free_slots.push_back(i);
+    hq.assign(nslots, MqQuery{});
+    for (auto& m : hq) m.done = 1;
+    qs.reserve(nslots);
+    h2d(qs.p, hq.data(), nslots, stream);
+    hist.reserve(static_cast<size_t>(nslots) * kMqHistStride);
+    mctl.reserve(1);
+    const MqCtl z{};
+    h2d(mctl.p, &z, 1, stream);
+    sel.reserve(nslots);
+    per_q.reserve(nslots);
+    st[0].reserve(first_cap);
+    batch.reserve(st[0].cap);
+    CPG_HIP(hipStreamSynchronize(stream));
+    ready = true;
+  }
+
+  int add_graph(const EdgeList& el) {
+    Graph G;
+    G.h = std::make_unique<cpg_handle>();
+    G.h->set_graph(el);
+    G.h->upload_graph();
+    const int64_t n = G.h->g.n, m = G.h->g.m;
+    G.index_bytes = 28 * n + 16 * m;  // vrec 16, devinfo 8, distf 4 per vertex; devs 16 per edge
+    graphs.push_back(std::move(G));
+    return static_cast<int>(graphs.size()) - 1;
+  }
+
+  int64_t submit(int graph, int64_t k, const uint8_t* mask) {
+    if (graph < 0 || graph >= static_cast<int>(graphs.size())) throw std::runtime_error("no such graph");
+    if (k < 0) throw std::runtime_error("k must be >= 0");
+    auto q = std::make_unique<Query>();
+    q->id = next_id++;
+    q->graph = graph;
+    q->k = k;
+    if (mask) q->mask.assign(mask, mask + graphs[graph].h->g.n);
+    q->t_submit = now_ms();
+    const int64_t id = q->id;
+    queries[id] = std::move(q);
+    queue.push_back(id);
+    stats.submitted++;
+    return id;
+  }
+
+  MqArgs args(const float* distf = nullptr) {
+    MqArgs a{};
+    a.nodes = st[cur].view();
+    a.nq = st[cur].nq.p;
+    a.distf = distf;
+    a.batch = batch.p;
+    a.hist = hist.p;
+    a.qs = qs.p;
+    a.ctl = mctl.p;
+    a.nq_count = nslots;
+    return a;
+  }
+
+  // The query's SFXT for its endpoint subset on its graph's handle, its
+  // roots, and the index buffers taken over from the handle.
+  void prepare(Query& q) {
+    Graph& G = graphs[q.graph];
+    cpg_handle& h = *G.h;
+    std::unique_ptr<IndexSet> ix;
+    if (!G.pool.empty()) {
+      ix = std::move(G.pool.back());
+      G.pool.pop_back();
+    } else {
+      ix = std::make_unique<IndexSet>();
+    }
+    h.vrec.swap(ix->vrec); h.devs.swap(ix->devs); h.devinfo.swap(ix->devinfo); h.distf.swap(ix->distf);
+    h.cache_enabled = false;
+    h.sfxt_valid = false;
+    const double t0 = now_ms();
+    h.run_sfxt(false, q.mask.empty() ? nullptr : q.mask.data());
+    stats.sfxt_ms += now_ms() - t0;
+    const int nsrc = static_cast<int>(h.g.srcs.size());
+    std::vector<int> sd(nsrc);
+    h.src_dist.reserve(std::max(1, nsrc));
+    launch_gather_int(h.dist.p, h.srcs.p, nsrc, h.src_dist.p, h.stream);
+    d2h(sd.data(), h.src_dist.p, nsrc, h.stream);
+    CPG_HIP(hipStreamSynchronize(h.stream));
+    std::vector<uint8_t> kp;
+    int64_t nroots = 0, dropped = 0;
+    float minc = 0.0f;
+    pick_roots(sd, true, 0.005f, kp, nroots, dropped, minc);
+    q.roots.clear();
+    for (int i = 0; i < nsrc; ++i)
+      if (kp[i]) q.roots.push_back(h.g.srcs[i]);
+    h.vrec.swap(ix->vrec); h.devs.swap(ix->devs); h.devinfo.swap(ix->devinfo); h.distf.swap(ix->distf);
+    h.sfxt_valid = false;  // the handle's index went to the query
+    MqQuery m{};
+    m.ukey = kKeyInf;
+    m.klo = q.roots.empty() ? 0u : float_key(minc);
+    m.shift = window_shift_bins(m.klo, kKeyInf, kMqBins);
+    m.k = static_cast<unsigned long long>(q.k);
+    m.batch_target = std::max<unsigned long long>(256, m.k / 8);
+    m.done = (q.roots.empty() || q.k == 0) ? 1 : 0;
+    m.vrec = ix->vrec.p;
+    m.devinfo = ix->devinfo.p;
+    m.devs = ix->devs.p;
+    m.mag = 2.0f * h.max_abs_distf + h.g.max_abs_wgt + 1.0f;
+    m.step_limit = h.step_limit();
+    m.all = 1;
+    q.init = m;
+    q.index = std::move(ix);
+  }
+
+  // Copies the live nodes (slots not done) into the other buffer set, of
+  // capacity at least want; the finished queries' slots become free.
+  void compact(uint64_t want) {
+    const double t0 = now_ms();
+    const uint64_t n = node_n;
+    const uint64_t nt = mq_compact_tiles(n);
+    tiles.reserve(nt + 1);
+    launch_zero_u32(tiles.p + nt, 1, stream);
+    MqArgs a = args();
+    launch_mq_count_live(a, n, tiles.p, stream);
+    launch_exclusive_scan_u32(tiles.p, nt + 1, stream);
+    unsigned live = 0;
+    d2h(&live, tiles.p + nt, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    Store& d = st[cur ^ 1];
+    d.reserve(std::max<uint64_t>({want, st[cur].cap, 1}));
+    newidx.reserve(std::max<uint64_t>(n, 1));
+    launch_mq_compact(a, n, d.view(), d.nq.p, tiles.p, newidx.p, live, grid, stream);
+    cur ^= 1;
+    node_n = live;
+    MqCtl c{};
+    c.node_n = live;
+    h2d(mctl.p, &c, 1, stream);
+    batch.reserve(st[cur].cap);
+    CPG_HIP(hipStreamSynchronize(stream));
+    for (int sl : dead_slots) free_slots.push_back(sl);
+    dead_slots.clear();
+    stats.compactions++;
+    stats.compact_ms += now_ms() - t0;
+  }
+
+  void admit() {
+    while (!queue.empty()) {
+      Query& q = *queries[queue.front()];
+      Graph& G = graphs[q.graph];
+      if (in_flight > 0 && index_bytes + G.index_bytes > budget) break;  // HBM admission
+      if (free_slots.empty()) {
+        if (dead_slots.empty()) break;
+        compact(st[cur].cap);
+      }
+      const uint64_t need = CPG_SERVER_NODES_PER_PATH * static_cast<uint64_t>(q.k) +
+                            G.h->g.srcs.size() + 65536;
+      if (node_n + need > st[cur].cap) {
+        compact(std::max(st[cur].cap, node_n + 2 * need));
+        if (node_n + need > st[cur].cap) compact(2 * (node_n + need));
+      }
+      queue.pop_front();
+      if (!q.index) prepare(q);
+      const int slot = free_slots.back();
+      free_slots.pop_back();
+      q.slot = slot;
+      q.state = 1;
+      q.t_admit = now_ms();
+      slot_query[slot] = q.id;
+      index_bytes += G.index_bytes;
+      stats.peak_index_bytes = std::max(stats.peak_index_bytes, index_bytes);
+      ++in_flight;
+      h2d(qs.p + slot, &q.init, 1, stream);
+      const int nr = static_cast<int>(q.roots.size());
+      if (nr > 0) {
+        root_v.reserve(nr);
+        root_q.reserve(nr);
+        std::vector<int> rq(nr, slot);
+        h2d(root_v.p, q.roots.data(), nr, stream);
+        h2d(root_q.p, rq.data(), nr, stream);
+        launch_mq_roots(args(q.index->distf.p), root_v.p, root_q.p, nr, stream);
+        node_n += static_cast<uint64_t>(nr);
+      }
+      CPG_HIP(hipStreamSynchronize(stream));
+      stats.admitted++;
+    }
+  }
+
+  // Sorted costs of the finished slots' paths (at or below each bound).
+  void extract(const std::vector<int>& fin) {
+    const double t0 = now_ms();
+    std::vector<uint8_t> hs(nslots, 0);
+    for (int sl : fin) hs[sl] = 1;
+    h2d(sel.p, hs.data(), nslots, stream);
+    const uint64_t nn = std::min<uint64_t>(node_n, st[cur].cap);
+    skey[0].reserve(std::max<uint64_t>(1, nn)); skey[1].reserve(std::max<uint64_t>(1, nn));
+    sval[0].reserve(std::max<uint64_t>(1, nn)); sval[1].reserve(std::max<uint64_t>(1, nn));
+    nsel.reserve(1);
+    CPG_HIP(hipMemsetAsync(nsel.p, 0, sizeof(unsigned long long), stream));
+    CPG_HIP(hipMemsetAsync(per_q.p, 0, sizeof(unsigned long long) * nslots, stream));
+    launch_mq_build_keys(args(), sel.p, nn, skey[0].p, sval[0].p, nsel.p, per_q.p, grid, stream);
+    unsigned long long nl = 0;
+    std::vector<unsigned long long> pq(nslots);
+    d2h(&nl, nsel.p, 1, stream);
+    d2h(pq.data(), per_q.p, nslots, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
+    SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
+    const int bytes[7] = {0, 1, 2, 3, 4, 5, 6};  // slot < 256: bits 48..55
+    const int res = radix_sort_pairs(sb, nl, bytes, 7, stream);
+    out_cost.reserve(std::max<uint64_t>(1, nl));
+    launch_mq_gather_costs(sval[res].p, st[cur].cost.p, nl, out_cost.p, stream);
+    std::vector<float> all(nl);
+    d2h(all.data(), out_cost.p, nl, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    const double t1 = now_ms();
+    uint64_t off = 0;
+    for (int sl = 0; sl < nslots; ++sl) {
+      if (!hs[sl]) continue;
+      Query& q = *queries[slot_query[sl]];
+      const uint64_t take = std::min<uint64_t>(pq[sl], static_cast<uint64_t>(q.k));
+      q.result.assign(all.begin() + off, all.begin() + off + take);
+      off += pq[sl];
+      q.state = 2;
+      q.t_done = t1;
+      completed.push_back(q.id);
+      Graph& G = graphs[q.graph];
+      G.pool.push_back(std::move(q.index));
+      index_bytes -= G.index_bytes;
+      --in_flight;
+      slot_query[sl] = -1;
+      dead_slots.push_back(sl);
+      stats.completed++;
+    }
+    stats.extract_ms += now_ms() - t0;
+  }
+
+  // A round overflowed the store: every in-flight query starts again from
+  // its roots (its index is kept) in an empty store of twice the capacity.
+  void restart_all() {
+    std::vector<int64_t> back;
+    for (int sl = 0; sl < nslots; ++sl) {
+      if (slot_query[sl] < 0) continue;
+      Query& q = *queries[slot_query[sl]];
+      q.state = 0;
+      q.slot = -1;
+      back.push_back(q.id);
+      index_bytes -= graphs[q.graph].index_bytes;
+      slot_query[sl] = -1;
+    }
+    for (auto it = back.rbegin(); it != back.rend(); ++it) queue.push_front(*it);
+    in_flight = 0;
+    for (auto& m : hq) m = MqQuery{}, m.done = 1;
+    h2d(qs.p, hq.data(), nslots, stream);
+    free_slots.clear();
+    dead_slots.clear();
+    for (int i = nslots - 1; i >= 0; --i) free_slots.push_back(i);
+    const uint64_t c = std::min<uint64_t>(2 * st[cur].cap, 0x7fffffffull);
+    st[cur].reserve(c);
+    batch.reserve(c);
+    node_n = 0;
+    const MqCtl z{};
+    h2d(mctl.p, &z, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    stats.restarts++;
+  }
+
+  // Admission, one round for every in-flight query, extraction of the
+  // finished ones.  Returns the queries still queued or in flight.
+  int64_t step() {
+    init();
+    admit();
+    if (in_flight > 0) {
+      const double t0 = now_ms();
+      MqArgs a = args();
+      launch_mq_round(a, false, grid, stream);
+      MqCtl mc{};
+      d2h(&mc, mctl.p, 1, stream);
+      d2h(hq.data(), qs.p, nslots, stream);
+      CPG_HIP(hipStreamSynchronize(stream));
+      stats.rounds++;
+      stats.round_ms += now_ms() - t0;
+      if (mc.err) throw std::runtime_error("served PFXT walk exceeded its step bound");
+      node_n = mc.node_n;
+      stats.peak_nodes = std::max<int64_t>(stats.peak_nodes, static_cast<int64_t>(std::min(node_n, st[cur].cap)));
+      if (mc.overflow) {
+        restart_all();
+      } else {
+        std::vector<int> fin;
+        for (int sl = 0; sl < nslots; ++sl)
+          if (slot_query[sl] >= 0 && hq[sl].done) fin.push_back(sl);
+        if (!fin.empty()) extract(fin);
+        if (!dead_slots.empty() && node_n > st[cur].cap / 2) compact(st[cur].cap);
+      }
+    }
+    stats.in_flight = in_flight;
+    stats.queued = static_cast<int64_t>(queue.size());
+    stats.node_cap = static_cast<int64_t>(st[cur].cap);
+    return stats.in_flight + stats.queued;
+  }
+
+  Query& get(int64_t id) {
+    auto it = queries.find(id);
+    if (it == queries.end()) throw std::runtime_error("no such query");
+    return *it->second;
   }
 };
 
@@ -1106,7 +1570,6 @@ int cpg_query(cpg_handle* h, int64_t k, const cpg_query_opts* o) {
         throw std::runtime_error("cpg_query_opts.struct_size mismatch");
       d = *o;
     }
-    if (d.endpoint_mask) throw std::runtime_error("endpoint_mask is not supported by this build");
     h->query(k, d);
     return 0;
   });
@@ -1323,6 +1786,97 @@ int cpg_batch_timings(cpg_batch* b, cpg_timing* out) {
   *out = b->timing;
   return 0;
 }
+
+cpg_server* cpg_server_create(int64_t budget_bytes, int max_in_flight, int64_t initial_nodes) {
+  return guarded<cpg_server*>(nullptr, [&]() {
+    return new cpg_server(budget_bytes, max_in_flight, initial_nodes);
+  });
+}
+
+void cpg_server_destroy(cpg_server* s) { delete s; }
+
+#define CPG_CHECK_SERVER(s, fail_value)                 \
+  if (!(s)) {                                           \
+    g_last_error = "null server";                       \
+    return fail_value;                                  \
+  }
+
+int cpg_server_add_graph(cpg_server* s, int n, int64_t m, const int* eu, const int* ev,
+                         const float* ew) {
+  CPG_CHECK_SERVER(s, -1)
+  return guarded<int>(-1, [&]() {
+    if (n < 0 || m < 0 || (m > 0 && (!eu || !ev || !ew))) throw std::runtime_error("bad edge list");
+    EdgeList el;
+    el.n = n;
+    el.u.assign(eu, eu + m);
+    el.v.assign(ev, ev + m);
+    el.w.assign(ew, ew + m);
+    return s->add_graph(el);
+  });
+}
+
+int64_t cpg_server_submit(cpg_server* s, int graph, int64_t k, const uint8_t* endpoint_mask) {
+  CPG_CHECK_SERVER(s, -1)
+  return guarded<int64_t>(-1, [&]() { return s->submit(graph, k, endpoint_mask); });
+}
+
+int64_t cpg_server_step(cpg_server* s) {
+  CPG_CHECK_SERVER(s, -1)
+  return guarded<int64_t>(-1, [&]() { return s->step(); });
+}
+
+int64_t cpg_server_poll(cpg_server* s, int64_t* ids, int64_t cap) {
+  CPG_CHECK_SERVER(s, -1)
+  return guarded<int64_t>(-1, [&]() {
+    const int64_t n = std::min<int64_t>(std::max<int64_t>(cap, 0), static_cast<int64_t>(s->completed.size()));
+    if (n > 0 && !ids) throw std::runtime_error("null ids");
+    for (int64_t i = 0; i < n; ++i) ids[i] = s->completed[i];
+    s->completed.erase(s->completed.begin(), s->completed.begin() + n);
+    return n;
+  });
+}
+
+int64_t cpg_server_get_costs(cpg_server* s, int64_t id, int64_t k, float* out) {
+  CPG_CHECK_SERVER(s, -1)
+  return guarded<int64_t>(-1, [&]() {
+    auto& q = s->get(id);
+    if (q.state != 2) throw std::runtime_error("query not finished");
+    const int64_t n = std::min<int64_t>(std::max<int64_t>(k, 0), static_cast<int64_t>(q.result.size()));
+    if (n && out) std::memcpy(out, q.result.data(), sizeof(float) * n);
+    return n;
+  });
+}
+
+int cpg_server_query_times(cpg_server* s, int64_t id, double* submit_ms, double* admit_ms,
+                           double* done_ms) {
+  CPG_CHECK_SERVER(s, -1)
+  return guarded<int>(-1, [&]() {
+    auto& q = s->get(id);
+    if (submit_ms) *submit_ms = q.t_submit;
+    if (admit_ms) *admit_ms = q.t_admit;
+    if (done_ms) *done_ms = q.t_done;
+    return 0;
+  });
+}
+
+int cpg_server_release(cpg_server* s, int64_t id) {
+  CPG_CHECK_SERVER(s, -1)
+  return guarded<int>(-1, [&]() {
+    auto& q = s->get(id);
+    if (q.state != 2) throw std::runtime_error("query not finished");
+    s->queries.erase(id);
+    return 0;
+  });
+}
+
+int cpg_server_get_stats(cpg_server* s, cpg_server_stats* out) {
+  CPG_CHECK_SERVER(s, -1)
+  if (!out) { g_last_error = "null argument"; return -1; }
+  *out = s->stats;
+  return 0;
+}
+
+double cpg_now_ms(void) { return now_ms(); }
 
 int cpg_cache_enable(cpg_handle* h, int enabled) {
   CPG_CHECK_HANDLE(h, -1)

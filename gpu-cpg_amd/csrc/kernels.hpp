@@ -31,6 +31,7 @@ struct SfxtArgs {
   const int* iadjp;
   const int2* iedge;  // fanin records {x, w_e bits}, in-CSR order
   const int* sinks;   // out-degree-0 vertices: level 0
+  const uint8_t* sink_mask;  // optional endpoint subset (n bytes, nonzero = chosen sink)
   unsigned long long* best;  // packed relax candidate per vertex (see sfxt.hip)
   int* deps;
   int* queue;
@@ -142,8 +143,17 @@ struct MqQuery {
   unsigned long long k;           // paths asked for
   unsigned long long batch_target;
   unsigned long long expanded;
-  int done;
+  int done;                       // nothing left to expand (or the slot is free)
   int rounds;
+  // the query's deviation index: the batched designs share the union's,
+  // a served query has its own (its endpoint subset's SFXT)
+  const int4* vrec;
+  const int2* devinfo;
+  const int4* devs;
+  float mag;                      // 2 max|distf| + max|w| + 1 (walk slack)
+  int step_limit;                 // bound on one walk's steps
+  int all;                        // the next round expands every unexpanded path
+  int pad;
 };
 
 struct MqCtl {
@@ -158,29 +168,36 @@ struct MqCtl {
 struct MqArgs {
   NodeStore nodes;
   int* nq;                        // query of each node
-  const int4* vrec;
-  const int2* devinfo;
-  const int4* devs;
-  const float* distf;
+  const float* distf;             // roots' costs (mq_roots)
   uint32_t* batch;
   unsigned* hist;                 // nq_count x kMqHistStride
   MqQuery* qs;
   MqCtl* ctl;
   int nq_count;
-  float max_abs_distf;
-  float max_abs_wgt;
-  int step_limit;
+  int pad;
 };
 
 void launch_mq_roots(const MqArgs& a, const int* root_v, const int* root_q, int nroots, hipStream_t s);
 // One round for every query (all: the batch takes every unexpanded path).
 void launch_mq_round(const MqArgs& a, bool all, int grid, hipStream_t s);
-void launch_mq_build_keys(const MqArgs& a, uint64_t nmax, uint64_t* key, uint32_t* val,
-                          unsigned long long* count, unsigned long long* per_q, int grid,
-                          hipStream_t s);
+// sel: one byte per query slot choosing the queries whose paths are taken
+// (null = every query).
+void launch_mq_build_keys(const MqArgs& a, const uint8_t* sel, uint64_t nmax, uint64_t* key,
+                          uint32_t* val, unsigned long long* count, unsigned long long* per_q,
+                          int grid, hipStream_t s);
 void launch_mq_gather_costs(const uint32_t* val, const float* cost, uint64_t n, float* out,
                             hipStream_t s);
 void launch_zero_u32(unsigned* p, uint64_t n, hipStream_t s);
+// Compaction of the shared store: live nodes (query slot not done) per
+// tile, then (after an exclusive scan of the tile counts) the ordered copy
+// into dst with newidx[old] = new slot (-1 for a dead node) and the
+// parents remapped.
+uint64_t mq_compact_tiles(uint64_t n);
+void launch_mq_count_live(const MqArgs& a, uint64_t n, unsigned* tile_cnt, hipStream_t s);
+void launch_mq_compact(const MqArgs& a, uint64_t n, const NodeStore& dst, int* dst_nq,
+                       const unsigned* tile_off, int* newidx, uint64_t n_live, int grid,
+                       hipStream_t s);
+void launch_exclusive_scan_u32(unsigned* p, uint64_t len, hipStream_t s);
 void launch_zero_active(MqCtl* ctl, hipStream_t s);
 
 // ---------------- BVSS deviation discovery ----------------

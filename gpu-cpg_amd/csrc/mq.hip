@@ -130,8 +130,10 @@ __global__ void __launch_bounds__(kMqSelBlock) mq_select_batch_kernel(MqArgs a, 
   if (qs->done) return;
   const unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
   unsigned long long total = 0;
-  const uint32_t b = mq_crossing(h + 2 * kMqBins, all ? ~0ull : qs->batch_target, &total);
+  const bool take_all = all || qs->all;
+  const uint32_t b = mq_crossing(h + 2 * kMqBins, take_all ? ~0ull : qs->batch_target, &total);
   if (threadIdx.x == 0) {
+    qs->all = 0;
     if (total == 0) {
       qs->done = 1;
     } else if (b >= static_cast<uint32_t>(kMqBins)) {
@@ -171,7 +173,6 @@ template <bool kFill>
 __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
   MqCtl* ctl = a.ctl;
   const uint64_t nb = ctl->batch_n;
-  const float mag = 2.0f * a.max_abs_distf + a.max_abs_wgt + 1.0f;
   unsigned long long hops = 0, devv = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < nb;
        base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -190,12 +191,12 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
     }
     const float U = key_float(qs.ukey);
     ChainWalk w;
-    w.start(i < nb, c, to, U, qs.ukey != kKeyInf, mag);
+    w.start(i < nb, c, to, U, qs.ukey != kKeyInf, qs.mag);
     unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
     while (__ballot(w.mode != 2)) {
       float cc = 0.0f;
       int cw = 0, from = 0;
-      const bool emit = w.step(a.vrec, a.devinfo, a.devs, U, !kFill, a.step_limit, &ctl->err,
+      const bool emit = w.step(qs.vrec, qs.devinfo, qs.devs, U, !kFill, qs.step_limit, &ctl->err,
                                &hops, &devv, &cc, &cw, &from);
       if (!kFill) {
         if (emit) {
@@ -251,9 +252,10 @@ __global__ void __launch_bounds__(kMqSelBlock) mq_select_bound_kernel(MqArgs a) 
 }
 
 // Sort input: each stored path with a nonzero key <= its query's bound as
-// (query << 48 | key << 16 | level) -> node.
-__global__ void __launch_bounds__(kMqBlock) mq_build_keys_kernel(MqArgs a, uint64_t* key,
-                                                                 uint32_t* val,
+// (query << 48 | key << 16 | level) -> node, for the queries sel marks
+// (every query when sel is null).
+__global__ void __launch_bounds__(kMqBlock) mq_build_keys_kernel(MqArgs a, const uint8_t* sel,
+                                                                 uint64_t* key, uint32_t* val,
                                                                  unsigned long long* count,
                                                                  unsigned long long* per_q) {
   const uint64_t n = a.ctl->node_n < a.nodes.cap ? a.ctl->node_n : a.nodes.cap;
@@ -266,7 +268,7 @@ __global__ void __launch_bounds__(kMqBlock) mq_build_keys_kernel(MqArgs a, uint6
     if (i < n) {
       q = a.nq[i];
       k = a.nodes.key[i];
-      take = k <= a.qs[q].ukey && !key_is_zero(k);
+      take = (!sel || sel[q]) && k <= a.qs[q].ukey && !key_is_zero(k);
     }
     const uint64_t slot = wave_reserve(count, take);
     if (take) {
@@ -275,6 +277,86 @@ __global__ void __launch_bounds__(kMqBlock) mq_build_keys_kernel(MqArgs a, uint6
       val[slot] = static_cast<uint32_t>(i);
       atomicAdd(&per_q[q], 1ull);
     }
+  }
+}
+
+// Compaction of the shared node store (served queries): a node is live
+// while its query's slot is not done.  Tiles of kCompactTile nodes keep
+// their order; parents are remapped through newidx afterwards.
+constexpr int kCompactChunks = 16;
+constexpr int kCompactTile = kMqBlock * kCompactChunks;
+
+__global__ void __launch_bounds__(kMqBlock) mq_count_live_kernel(MqArgs a, uint64_t n,
+                                                                 unsigned* tile_cnt) {
+  __shared__ unsigned s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const uint64_t t = blockIdx.x;
+  unsigned c = 0;
+  for (int j = 0; j < kCompactChunks; ++j) {
+    const uint64_t i = t * kCompactTile + static_cast<uint64_t>(j) * kMqBlock + threadIdx.x;
+    if (i < n && !a.qs[a.nq[i]].done) ++c;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (__lane_id() == 0) atomicAdd(&s_cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_cnt[t] = s_cnt;
+}
+
+__global__ void __launch_bounds__(kMqBlock) mq_compact_kernel(MqArgs a, uint64_t n, NodeStore dst,
+                                                              int* dst_nq, const unsigned* tile_off,
+                                                              int* newidx) {
+  constexpr int kW = kMqBlock / 64;
+  __shared__ unsigned s_wave[kW];
+  __shared__ unsigned s_run;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_run = 0;
+  __syncthreads();
+  const uint64_t t = blockIdx.x;
+  const uint64_t base = tile_off[t];
+  for (int j = 0; j < kCompactChunks; ++j) {
+    const uint64_t i = t * kCompactTile + static_cast<uint64_t>(j) * kMqBlock + threadIdx.x;
+    int q = 0;
+    bool live = false;
+    if (i < n) {
+      q = a.nq[i];
+      live = !a.qs[q].done;
+    }
+    const uint64_t m = __ballot(live);
+    const unsigned rank = static_cast<unsigned>(__popcll(m & lanemask_lt()));
+    if (lane == 0) s_wave[wave] = static_cast<unsigned>(__popcll(m));
+    __syncthreads();
+    unsigned off = s_run;
+    for (int w = 0; w < wave; ++w) off += s_wave[w];
+    if (live) {
+      const uint64_t d = base + off + rank;
+      dst.cost[d] = a.nodes.cost[i];
+      dst.to[d] = a.nodes.to[i];
+      dst.from[d] = a.nodes.from[i];
+      dst.parent[d] = a.nodes.parent[i];
+      dst.level[d] = a.nodes.level[i];
+      dst.key[d] = a.nodes.key[i];
+      dst.flag[d] = a.nodes.flag[i];
+      dst_nq[d] = q;
+      newidx[i] = static_cast<int>(d);
+    } else if (i < n) {
+      newidx[i] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned tot = 0;
+      for (int w = 0; w < kW; ++w) tot += s_wave[w];
+      s_run += tot;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void mq_remap_parents_kernel(int* parent, const int* newidx, uint64_t n) {
+  for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+       j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const int p = parent[j];
+    if (p >= 0) parent[j] = newidx[p];
   }
 }
 
@@ -333,11 +415,36 @@ void launch_mq_round(const MqArgs& a, bool all, int grid, hipStream_t s) {
   CPG_HIP(hipGetLastError());
 }
 
-void launch_mq_build_keys(const MqArgs& a, uint64_t nmax, uint64_t* key, uint32_t* val,
-                          unsigned long long* count, unsigned long long* per_q, int grid,
-                          hipStream_t s) {
+void launch_mq_build_keys(const MqArgs& a, const uint8_t* sel, uint64_t nmax, uint64_t* key,
+                          uint32_t* val, unsigned long long* count, unsigned long long* per_q,
+                          int grid, hipStream_t s) {
   if (nmax == 0) return;
-  hipLaunchKernelGGL(mq_build_keys_kernel, dim3(grid), dim3(kMqBlock), 0, s, a, key, val, count, per_q);
+  hipLaunchKernelGGL(mq_build_keys_kernel, dim3(grid), dim3(kMqBlock), 0, s, a, sel, key, val,
+                     count, per_q);
+  CPG_HIP(hipGetLastError());
+}
+
+uint64_t mq_compact_tiles(uint64_t n) { return (n + kCompactTile - 1) / kCompactTile; }
+
+void launch_mq_count_live(const MqArgs& a, uint64_t n, unsigned* tile_cnt, hipStream_t s) {
+  const uint64_t nt = mq_compact_tiles(n);
+  if (nt == 0) return;
+  hipLaunchKernelGGL(mq_count_live_kernel, dim3(static_cast<unsigned>(nt)), dim3(kMqBlock), 0, s, a,
+                     n, tile_cnt);
+  CPG_HIP(hipGetLastError());
+}
+
+void launch_mq_compact(const MqArgs& a, uint64_t n, const NodeStore& dst, int* dst_nq,
+                       const unsigned* tile_off, int* newidx, uint64_t n_live, int grid,
+                       hipStream_t s) {
+  const uint64_t nt = mq_compact_tiles(n);
+  if (nt == 0) return;
+  hipLaunchKernelGGL(mq_compact_kernel, dim3(static_cast<unsigned>(nt)), dim3(kMqBlock), 0, s, a, n,
+                     dst, dst_nq, tile_off, newidx);
+  CPG_HIP(hipGetLastError());
+  if (n_live == 0) return;
+  hipLaunchKernelGGL(mq_remap_parents_kernel, dim3(grid), dim3(kMqBlock), 0, s, dst.parent, newidx,
+                     n_live);
   CPG_HIP(hipGetLastError());
 }
 

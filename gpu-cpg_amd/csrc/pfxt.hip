@@ -133,14 +133,22 @@ __global__ void __launch_bounds__(kBlock) hist_nodes_kernel(NodeStore nodes, con
     h_max[j] = 0;
   }
   __syncthreads();
+  // [lo, nn) within the buffer: after a node-store overflow node_n and
+  // fill_node_base exceed the capacity, and the vector loop below reads
+  // the group of four holding lo, so lo must not pass nn
   uint64_t nn, lo = 0;
   if (mode == 2) {
-    nn = ctl->child_n < ckeys_cap ? ctl->child_n : ckeys_cap;
+    const uint64_t c = ctl->child_n;
+    nn = c < ckeys_cap ? c : ckeys_cap;
   } else {
-    nn = ctl->node_n < nodes.cap ? ctl->node_n : nodes.cap;
-    if (mode == 1) lo = ctl->fill_node_base;
+    const uint64_t c = ctl->node_n;
+    nn = c < nodes.cap ? c : nodes.cap;
+    if (mode == 1) {
+      const uint64_t b = ctl->fill_node_base;
+      lo = b < nn ? b : nn;
+    }
   }
-  if (ctl->done) nn = lo;  // nothing left: write empty partials
+  if (ctl->done) lo = nn = 0;  // nothing left: write empty partials
   const uint32_t uk = ctl->ukey;
   const uint32_t klo = ctl->klo, shift = ctl->shift;
   const uint4* key4 = reinterpret_cast<const uint4*>(mode == 2 ? ckeys : nodes.key);

@@ -208,6 +208,12 @@ int grid_for(uint64_t n, int block) {
 
 uint64_t sort_tiles(uint64_t n) { return (n + kTile - 1) / kTile; }
 
+void launch_exclusive_scan_u32(unsigned* p, uint64_t len, hipStream_t s) {
+  if (len == 0) return;
+  hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, p, len);
+  CPG_HIP(hipGetLastError());
+}
+
 int radix_sort_pairs(SortBufs& b, uint64_t n, const int* bytes, int nbytes, hipStream_t s) {
   int cur = 0;
   if (n <= 1) return cur;

@@ -15,6 +15,13 @@
 // with more blocks, and levels hold a few thousand vertices) and does no
 // more than the relax, the successor and the fanin release.
 //
+// Endpoint subset (an extension; the reference has no such option): with a
+// sink mask, only the chosen sinks start at distance 0.  The other sinks,
+// and every vertex whose fanouts are all unreachable, keep dist INT_MAX
+// and succ -1 and offer no candidates; levels are unchanged.  A path then
+// ends at a chosen sink, as the reference's own unreachable test (dist[w]
+// != INT_MAX, gpucpg/gpucpg.cu:2662-2668) already implies for deviations.
+//
 // The deviation index is this engine's own structure for the PFXT walk,
 // built afterwards by fully parallel passes:
 //   devs     per vertex u, its deviation edges (u->w, w != succ[u], dist[w]
@@ -188,17 +195,24 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
       const int u = a.queue[i];
       const unsigned long long key = a.best[u];
       const int b = a.iadjp[u], e = a.iadjp[u + 1];
+      // level 0: a sink outside the endpoint subset is unreachable; later
+      // levels: no candidate arrived (every fanout unreachable)
       int du = 0, s = -1;
-      if (d > 0) {
+      if (d == 0) {
+        if (a.sink_mask && !a.sink_mask[u]) du = kIntMax;
+      } else if (key == ~0ull) {
+        du = kIntMax;
+      } else {
         du = unpack_cand(key);
         s = unpack_succ(key);
       }
+      const bool reach = du != kIntMax;
       const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
       a.dist[u] = du;
       a.succ[u] = s;
       a.level[u] = d;
       a.distf[u] = df;
-      local_max = fmaxf(local_max, fabsf(df));
+      if (reach) local_max = fmaxf(local_max, fabsf(df));
       for (int j = b; j < e; j += kFaninUnroll) {
         // kFaninUnroll fanin records; past the end the last one is loaded
         // again (so every load is unconditional) and its atomics are skipped
@@ -211,7 +225,7 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
           old[q] = 0;
           if (j + q < e) {
             const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
-            atomicMin(&a.best[r[q].x], pack_cand(du + wi, u));
+            if (reach) atomicMin(&a.best[r[q].x], pack_cand(du + wi, u));
             old[q] = atomicSub(&a.deps[r[q].x], 1);
           }
         }
@@ -250,7 +264,7 @@ __global__ void __launch_bounds__(256) sfxt_devs_kernel(SfxtArgs a) {
     bool sorted = true;
     if (u < a.n) {
       const int b = a.oadjp[u], e = a.oadjp[u + 1];
-      if (a.level[u] < 0) {
+      if (a.level[u] < 0 || a.dist[u] == kIntMax) {  // unsettled or reaches no chosen sink
         a.vrec[u] = default_vrec();
         a.devinfo[u] = make_int2(b, 0);
       } else {

@@ -173,6 +173,51 @@ int64_t cpg_batch_get_costs(cpg_batch* b, int q, int64_t k, float* out);
 /* Whole-batch timings (sfxt/pfxt/select/query ms, rounds, generated). */
 int cpg_batch_timings(cpg_batch* b, cpg_timing* out);
 
+/* ---- served queries ----
+ * No counterpart in the reference (its CpGen answers one query at a time).
+ * Resident graphs and a stream of top-K queries, each with an optional
+ * endpoint subset; queries are admitted between rounds while the device
+ * bytes of the in-flight queries' deviation indexes stay within the budget
+ * and a slot is free, and the PFXT rounds of every in-flight query run
+ * together.  Each result equals cpg_query's for the same graph, K and
+ * endpoint mask (default options).  Not thread-safe: one thread drives a
+ * server. */
+typedef struct cpg_server cpg_server;
+typedef struct {
+  int64_t submitted, admitted, completed, rounds, compactions, restarts;
+  int64_t peak_nodes;        /* nodes stored at once (shared store)          */
+  int64_t node_cap;          /* current store capacity                       */
+  int64_t peak_index_bytes;  /* in-flight deviation indexes, device bytes    */
+  int64_t in_flight, queued;
+  double sfxt_ms, round_ms, extract_ms, compact_ms;  /* host wall time      */
+} cpg_server_stats;
+/* budget_bytes: device bytes for in-flight indexes (0 = 64 GiB);
+ * max_in_flight: query slots, at most 256 (0 = 64); initial_nodes: first
+ * capacity of the shared node store (0 = 2^24; it grows as needed). */
+cpg_server* cpg_server_create(int64_t budget_bytes, int max_in_flight, int64_t initial_nodes);
+void cpg_server_destroy(cpg_server* s);
+/* Makes a graph resident (edge list as cpg_load_edges); returns its id. */
+int cpg_server_add_graph(cpg_server* s, int n, int64_t m, const int* eu, const int* ev,
+                         const float* ew);
+/* Queues a query; endpoint_mask: NULL or n bytes (nonzero = chosen sink),
+ * copied.  Returns the query id (> 0). */
+int64_t cpg_server_submit(cpg_server* s, int graph, int64_t k, const uint8_t* endpoint_mask);
+/* Admission, one round of every in-flight query, extraction of the
+ * finished ones; returns the number of queries still queued or in flight. */
+int64_t cpg_server_step(cpg_server* s);
+/* Ids of the queries finished since the last poll (up to cap). */
+int64_t cpg_server_poll(cpg_server* s, int64_t* ids, int64_t cap);
+/* First k costs of a finished query (ascending, exact); -1 if not done. */
+int64_t cpg_server_get_costs(cpg_server* s, int64_t id, int64_t k, float* out);
+/* Submit/admit/done times of a query in ms on the engine's steady clock. */
+int cpg_server_query_times(cpg_server* s, int64_t id, double* submit_ms, double* admit_ms,
+                           double* done_ms);
+/* Forgets a finished query. */
+int cpg_server_release(cpg_server* s, int64_t id);
+int cpg_server_get_stats(cpg_server* s, cpg_server_stats* out);
+/* The engine's steady clock in ms (for arrival schedules). */
+double cpg_now_ms(void);
+
 /* ---- static cache, gpucpg/gpucpg.cuh:186-189 ---- */
 int cpg_cache_enable(cpg_handle* h, int enabled);
 int cpg_cache_clear(cpg_handle* h);

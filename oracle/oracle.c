@@ -202,26 +202,36 @@ int orc_levelize(const orc_graph* g, int* level, int* queue, int* lvlp,
 }
 
 /* a4: relax_bu_step gpucpg/gpucpg.cu:1167-1187, loop :17933-17961,
- * init :17035-17039 (sinks 0, others INT_MAX). */
-void orc_relax(const orc_graph* g, const int* queue, const int* lvlp,
-               int num_levels, int* dist) {
+ * init :17035-17039 (sinks 0, others INT_MAX).  With sink_mask (n bytes,
+ * an endpoint subset: an extension the reference does not have), only
+ * the chosen sinks start at 0; a fanout at INT_MAX offers no candidate,
+ * so a vertex reaching no chosen sink stays at INT_MAX. */
+void orc_relax_masked(const orc_graph* g, const int* queue, const int* lvlp,
+                      int num_levels, const unsigned char* sink_mask, int* dist) {
   for (int v = 0; v < g->n; ++v) dist[v] = ORC_INT_MAX;
   if (num_levels > 0)
-    for (int i = lvlp[0]; i < lvlp[1]; ++i) dist[queue[i]] = 0;
+    for (int i = lvlp[0]; i < lvlp[1]; ++i)
+      if (!sink_mask || sink_mask[queue[i]]) dist[queue[i]] = 0;
   for (int d = 1; d < num_levels; ++d) {
     for (int i = lvlp[d]; i < lvlp[d + 1]; ++i) {
       const int v = queue[i];
       int best = ORC_INT_MAX;
       for (int e = g->oadjp[v]; e < g->oadjp[v + 1]; ++e) {
         const int w = g->oadj[e];
+        if (dist[w] == ORC_INT_MAX) continue;
         const int wgt = (int)(g->owgt[e] * (float)ORC_SCALE_UP);
         const long long cand = (long long)dist[w] + wgt;
-        const int c = (int)cand;  /* fanouts of a levelized vertex are finite */
+        const int c = (int)cand;
         if (c < best) best = c;
       }
       dist[v] = best;
     }
   }
+}
+
+void orc_relax(const orc_graph* g, const int* queue, const int* lvlp,
+               int num_levels, int* dist) {
+  orc_relax_masked(g, queue, lvlp, num_levels, NULL, dist);
 }
 
 /* a5: update_successors_bu gpucpg/gpucpg.cu:2606-2628 -- the largest
@@ -245,18 +255,23 @@ void orc_distf(int n, const int* dist, float* distf) {
   for (int v = 0; v < n; ++v) distf[v] = (float)dist[v] / (float)ORC_SCALE_UP;
 }
 
-int orc_sfxt(const orc_graph* g, int* dist, int* succ, int* level,
-             int* num_levels) {
+int orc_sfxt_masked(const orc_graph* g, const unsigned char* sink_mask, int* dist,
+                    int* succ, int* level, int* num_levels) {
   const int n = g->n;
   int* queue = malloc(sizeof(int) * (size_t)(n ? n : 1));
   int* lvlp = malloc(sizeof(int) * ((size_t)n + 2));
   if (!queue || !lvlp) { free(queue); free(lvlp); return fail("out of memory"); }
   int cnt = orc_levelize(g, level, queue, lvlp, num_levels);
-  orc_relax(g, queue, lvlp, *num_levels, dist);
+  orc_relax_masked(g, queue, lvlp, *num_levels, sink_mask, dist);
   orc_successors(g, dist, succ);
   free(queue);
   free(lvlp);
   return cnt;
+}
+
+int orc_sfxt(const orc_graph* g, int* dist, int* succ, int* level,
+             int* num_levels) {
+  return orc_sfxt_masked(g, NULL, dist, succ, level, num_levels);
 }
 
 /* a17: candidate_slack gpucpg/gpucpg/tc_pfxt_candidates.cuh:651-660;

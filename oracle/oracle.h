@@ -15,6 +15,8 @@
  *                       (memoised DFS in the tests) and the known answer
  *                       {3,3,2,1,2,1,0,0} of unittests/tc_sfxt_bvss.cu:153-172.
  *   orc_relax           independent Dijkstra over int-scaled weights.
+ *   orc_relax_masked    the same Dijkstra started from the chosen sinks only;
+ *                       a mask of every sink equals orc_relax.
  *   orc_successors      brute-force max-id arg-min per vertex.
  *   orc_distf           IEEE float division (numpy float32 divide).
  *   orc_candidate_cost  known answer -0.75 of unittests/tc_pfxt_candidates.cu:85-90.
@@ -98,6 +100,11 @@ void orc_successors(const orc_graph* g, const int* dist, int* succ);
 void orc_distf(int n, const int* dist, float* distf);
 int orc_sfxt(const orc_graph* g, int* dist, int* succ, int* level,
              int* num_levels);
+/* endpoint subset: sink_mask n bytes, nonzero = chosen sink (NULL = all) */
+void orc_relax_masked(const orc_graph* g, const int* queue, const int* lvlp,
+                      int num_levels, const unsigned char* sink_mask, int* dist);
+int orc_sfxt_masked(const orc_graph* g, const unsigned char* sink_mask, int* dist,
+                    int* succ, int* level, int* num_levels);
 
 /* ---- candidate cost (a10/a17) ---- */
 float orc_candidate_cost(float parent, int src_dist, int dst_dist, float wgt);

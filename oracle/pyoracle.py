@@ -53,6 +53,7 @@ _SIGS = {
     "orc_successors": (None, [_PG, _PI, _PI]),
     "orc_distf": (None, [ctypes.c_int, _PI, _PF]),
     "orc_sfxt": (ctypes.c_int, [_PG, _PI, _PI, _PI, _PI]),
+    "orc_sfxt_masked": (ctypes.c_int, [_PG, ctypes.c_void_p, _PI, _PI, _PI, _PI]),
     "orc_candidate_cost": (ctypes.c_float, [ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float]),
     "orc_split_inc_amount": (ctypes.c_float, [ctypes.c_float]),
     "orc_classify": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
@@ -158,13 +159,20 @@ class OracleGraph:
         cnt = lib().orc_levelize(ctypes.byref(self.g), _ip(level), _ip(queue), _ip(lvlp), ctypes.byref(L))
         return level[:n], queue[:cnt], lvlp[:L.value + 1], L.value
 
-    def sfxt(self):
+    def sfxt(self, sink_mask=None):
+        """sink_mask: optional n-byte endpoint subset (nonzero = chosen sink)."""
         n = self.g.n
         dist = np.zeros(max(n, 1), np.int32)
         succ = np.zeros(max(n, 1), np.int32)
         level = np.zeros(max(n, 1), np.int32)
         L = ctypes.c_int(0)
-        lib().orc_sfxt(ctypes.byref(self.g), _ip(dist), _ip(succ), _ip(level), ctypes.byref(L))
+        if sink_mask is None:
+            lib().orc_sfxt(ctypes.byref(self.g), _ip(dist), _ip(succ), _ip(level), ctypes.byref(L))
+        else:
+            m = np.ascontiguousarray(sink_mask, dtype=np.uint8)
+            assert m.shape == (n,)
+            lib().orc_sfxt_masked(ctypes.byref(self.g), m.ctypes.data, _ip(dist), _ip(succ),
+                                  _ip(level), ctypes.byref(L))
         return dist[:n], succ[:n], level[:n], L.value
 
     def topk(self, dist, succ, k, root_tie_quirk=True):

@@ -10,7 +10,11 @@
 // that grid-wide barriers made of atomics can complete.
 #include "hip/hip_runtime.h"
 
+#include <dlfcn.h>
+#include <execinfo.h>
+#include <signal.h>
 #include <sys/mman.h>
+#include <unistd.h>
 #include <ucontext.h>
 
 #include <atomic>
@@ -41,7 +45,7 @@ struct Stack {
   Stack& operator=(const Stack&) = delete;
   Stack(Stack&& o) noexcept : p(o.p) { o.p = nullptr; }
   ~Stack() {
-    if (!p) return;
+    if (!p) return;  // stacks (and their guard pages) stay mapped in the pool
     std::lock_guard<std::mutex> lk(g_stack_mu);
     g_stack_pool.push_back(p);
   }
@@ -55,9 +59,11 @@ struct Stack {
         return p;
       }
     }
-    void* m = mmap(nullptr, kStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    // one inaccessible page below the stack: an overflow faults there
+    void* m = mmap(nullptr, kStack + 4096, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
     if (m == MAP_FAILED) std::abort();
-    p = static_cast<char*>(m);
+    mprotect(m, 4096, PROT_NONE);
+    p = static_cast<char*>(m) + 4096;
     return p;
   }
 };
@@ -113,7 +119,21 @@ void dump_and_abort(Block& B, const char* why) {
   std::abort();
 }
 
+// The fault reporter runs on its own per-thread stack (a fiber's stack
+// may be the one that overflowed).
+void ensure_altstack() {
+  static thread_local bool done = false;
+  if (done) return;
+  done = true;
+  static thread_local char alt[64 * 1024];
+  stack_t ss{};
+  ss.ss_sp = alt;
+  ss.ss_size = sizeof alt;
+  sigaltstack(&ss, nullptr);
+}
+
 void run_block(Block& B) {
+  ensure_altstack();
   const int n = static_cast<int>(B.dim.x);
   B.lanes.resize(n);
   B.waves.assign((n + 63) / 64, Wave{});
@@ -274,7 +294,88 @@ bool guard_on() {
 }
 std::mutex g_map_mu;
 std::map<void*, std::pair<void*, size_t>> g_maps;  // buffer -> (mapping, length)
+std::vector<std::pair<uintptr_t, uintptr_t>> g_freed;  // [begin, end) of freed buffers
 }  // namespace
+
+void simt::check_arg(const void* kernel, int index, const void* arg, size_t bytes) {
+  if (!guard_on()) return;
+  std::lock_guard<std::mutex> lk(g_map_mu);
+  for (size_t o = 0; o + 8 <= bytes; o += 8) {
+    uintptr_t v;
+    std::memcpy(&v, static_cast<const char*>(arg) + o, 8);
+    if (v < 4096) continue;
+    bool live = false;
+    for (auto& kv : g_maps) {
+      const uintptr_t m = reinterpret_cast<uintptr_t>(kv.second.first);
+      if (v >= m && v < m + kv.second.second) { live = true; break; }
+    }
+    if (live) continue;
+    for (auto& f : g_freed) {
+      if (v >= f.first && v < f.second) {
+        Dl_info di{};
+        dladdr(kernel, &di);
+        std::fprintf(stderr, "simt: kernel %s argument %d word at +%zu = %#lx points into a freed "
+                     "buffer [%#lx, %#lx)\n", di.dli_sname ? di.dli_sname : "?", index, o,
+                     static_cast<unsigned long>(v), static_cast<unsigned long>(f.first),
+                     static_cast<unsigned long>(f.second));
+        std::abort();
+      }
+    }
+  }
+}
+// With SIMT_GUARD=1, a fault reports the faulting address, the buffer
+// whose guard page it hit (its size) and the native backtrace.
+void guard_fault(int, siginfo_t* si, void*) {
+  char buf[256];
+  const uintptr_t a = reinterpret_cast<uintptr_t>(si->si_addr);
+  int len = std::snprintf(buf, sizeof buf, "simt: fault at %#lx (lane stack %p)\n",
+                          static_cast<unsigned long>(a),
+                          simt::t_lane ? static_cast<void*>(simt::t_lane->stack.p) : nullptr);
+  (void)!write(2, buf, len);
+  uintptr_t best = 0, best_sz = 0;
+  for (auto& kv : g_maps) {
+    const uintptr_t b = reinterpret_cast<uintptr_t>(kv.first);
+    const uintptr_t m = reinterpret_cast<uintptr_t>(kv.second.first);
+    if (b <= a && b > best) {
+      best = b;
+      best_sz = m + kv.second.second - 4096 - b;
+    }
+  }
+  len = std::snprintf(buf, sizeof buf, "simt: nearest buffer below: %#lx, %lu bytes, offset %+ld\n",
+                      static_cast<unsigned long>(best), static_cast<unsigned long>(best_sz),
+                      static_cast<long>(a - best));
+  (void)!write(2, buf, len);
+  for (auto& kv : g_maps) {
+    const uintptr_t b = reinterpret_cast<uintptr_t>(kv.first);
+    const uintptr_t m = reinterpret_cast<uintptr_t>(kv.second.first);
+    if (a >= m && a < m + kv.second.second) {
+      len = std::snprintf(buf, sizeof buf, "simt: buffer %#lx, %lu bytes to its guard, offset %+ld\n",
+                          static_cast<unsigned long>(b), static_cast<unsigned long>(m + kv.second.second - 4096 - b),
+                          static_cast<long>(a - b));
+      (void)!write(2, buf, len);
+    }
+  }
+  for (auto& kv : g_maps) {
+    len = std::snprintf(buf, sizeof buf, "simt: live buffer %#lx mapping %#lx +%lu\n",
+                        static_cast<unsigned long>(reinterpret_cast<uintptr_t>(kv.first)),
+                        static_cast<unsigned long>(reinterpret_cast<uintptr_t>(kv.second.first)),
+                        static_cast<unsigned long>(kv.second.second));
+    (void)!write(2, buf, len);
+  }
+  void* fr[64];
+  const int nf = backtrace(fr, 64);
+  backtrace_symbols_fd(fr, nf, 2);
+  _exit(139);
+}
+struct GuardInit {
+  GuardInit() {
+    if (!guard_on()) return;
+    struct sigaction sa {};
+    sa.sa_sigaction = guard_fault;
+    sa.sa_flags = SA_SIGINFO | SA_ONSTACK;
+    sigaction(SIGSEGV, &sa, nullptr);
+  }
+} g_guard_init;
 hipError_t simt_malloc(void** p, size_t bytes) {
   if (!bytes) bytes = 1;
   if (!guard_on()) {
@@ -303,6 +404,9 @@ hipError_t hipFree(void* p) {
   std::lock_guard<std::mutex> lk(g_map_mu);
   auto it = g_maps.find(p);
   if (it == g_maps.end()) std::abort();
+  const uintptr_t b = reinterpret_cast<uintptr_t>(p);
+  const uintptr_t e = reinterpret_cast<uintptr_t>(it->second.first) + it->second.second - 4096;
+  g_freed.emplace_back(b, e);
   munmap(it->second.first, it->second.second);
   g_maps.erase(it);
   return hipSuccess;

@@ -185,6 +185,27 @@ def test_c1_node_store_overflow_restarts_exactly(k, max_nodes):
     assert_bits_equal(want, g.get_slacks(k), f"C1 overflow K={k}")
 
 
+@pytest.mark.parametrize("design", [2, 9])
+def test_batch_designs_with_store_overflow(design):
+    """Regression: designs of the 256-design batch workload whose single
+    query overflows the default node store inside a group of rounds.  A
+    later round of that group that finds nothing left to expand
+    histogrammed the children of the round from fill_node_base (then past
+    the store's capacity) and read one group of keys beyond the key buffer;
+    the range is now clamped to the store."""
+    n, u, v, w = cpgamd.gen_preset(4, design)
+    k = int(cpgamd.preset_k(4, design))
+    say(f"case batch design {design} single query K={k} (store overflow)")
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, _, _ = og.sfxt()
+    want, _ = og.topk(dist, succ, k)
+    for rep in range(3):
+        g = engine_for(n, u, v, w)
+        g.report_paths(k)
+        assert g.timings()["retries"] >= 1
+        assert_bits_equal(want, g.get_slacks(k), f"batch design {design} rep {rep}")
+
+
 @pytest.mark.parametrize("probe", [1, -1])
 @pytest.mark.parametrize("name", ["simple.txt", "dac21_paper_example.txt", "tv80_unit_wgts.txt"])
 def test_reference_benchmarks_bound_probe_modes(golden_dir, name, probe):

@@ -54,8 +54,9 @@ def py_levels(n, adjp, adj):
     return np.array([lvl(v) for v in range(n)], np.int32)
 
 
-def py_dist(n, adjp, adj, wgt):
-    """Shortest int-scaled distance to any sink (Dijkstra on the reverse)."""
+def py_dist(n, adjp, adj, wgt, sink_mask=None):
+    """Shortest int-scaled distance to any sink (Dijkstra on the reverse);
+    with sink_mask, to any chosen sink."""
     import heapq
     radj = [[] for _ in range(n)]
     for u in range(n):
@@ -64,7 +65,7 @@ def py_dist(n, adjp, adj, wgt):
     dist = [INT_MAX] * n
     pq = []
     for v in range(n):
-        if adjp[v + 1] == adjp[v]:
+        if adjp[v + 1] == adjp[v] and (sink_mask is None or sink_mask[v]):
             dist[v] = 0
             pq.append((0, v))
     heapq.heapify(pq)
@@ -223,6 +224,75 @@ def test_sfxt_matches_closed_forms(seed):
     np.testing.assert_array_equal(dist.astype(np.int64), pd)
     np.testing.assert_array_equal(succ, py_succ(n, adjp, adj, wgt, pd))
     np.testing.assert_array_equal(po.distf(dist), (dist.astype(F32) / F32(10000)).astype(F32))
+
+
+def random_sink_mask(rng, n, adjp, frac):
+    sinks = [v for v in range(n) if adjp[v + 1] == adjp[v]]
+    pick = rng.choice(sinks, size=max(1, int(round(frac * len(sinks)))), replace=False)
+    m = np.zeros(n, np.uint8)
+    m[pick] = 1
+    return m
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_masked_sfxt_matches_closed_forms(seed):
+    """Endpoint subset: Dijkstra from the chosen sinks only; the successor
+    rule over reachable fanouts; a mask of every sink equals no mask."""
+    n = 40
+    rng = np.random.default_rng(500 + seed)
+    edges = random_dag(seed, n, 90)
+    adjp, adj, wgt, iadjp, iadj = csr_from_edges(n, edges)
+    g = og_from(n, edges)
+    full = g.sfxt()
+    allm = np.array([1 if adjp[v + 1] == adjp[v] else 0 for v in range(n)], np.uint8)
+    for a, b in zip(full, g.sfxt(allm)):
+        np.testing.assert_array_equal(a, b)
+    mask = random_sink_mask(rng, n, adjp, [0.1, 0.3, 0.6][seed % 3])
+    dist, succ, level, L = g.sfxt(mask)
+    np.testing.assert_array_equal(level, full[2])
+    pd = py_dist(n, adjp, adj, wgt, mask)
+    np.testing.assert_array_equal(dist.astype(np.int64), pd)
+    np.testing.assert_array_equal(succ, py_succ(n, adjp, adj, wgt, pd))
+    assert (dist == INT_MAX).any()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_masked_topk_paths_end_at_chosen_sinks(seed):
+    """The masked top-K equals the brute force over the masked SFXT, and
+    its costs are the weight sums of exactly the source -> chosen-sink
+    paths (equal up to the 1e-4 truncation of the int-scaled distances
+    and binary32 rounding)."""
+    n = 14
+    rng = np.random.default_rng(900 + seed)
+    seen, edges = set(), []
+    for e in random_dag(300 + seed, n, 32, parallel=False):  # no parallel edges
+        if (e[0], e[1]) not in seen:
+            seen.add((e[0], e[1]))
+            edges.append(e)
+    adjp, adj, wgt, iadjp, iadj = csr_from_edges(n, edges)
+    g = og_from(n, edges)
+    mask = random_sink_mask(rng, n, adjp, 0.5)
+    dist, succ, _, _ = g.sfxt(mask)
+    allp = py_all_paths(n, adjp, adj, wgt, iadjp, dist.astype(np.int64), succ)
+    got, _ = g.topk(dist, succ, len(allp) + 3, root_tie_quirk=False)
+    np.testing.assert_array_equal(got, allp)
+    np.testing.assert_array_equal(g.all_paths(dist, succ), allp)
+    sums = []
+
+    def walk(u, acc, hops):
+        if adjp[u + 1] == adjp[u]:
+            if mask[u]:
+                sums.append((acc, hops))
+            return
+        for e in range(adjp[u], adjp[u + 1]):
+            walk(int(adj[e]), acc + float(wgt[e]), hops + 1)
+    for v in range(n):
+        if iadjp[v + 1] == iadjp[v]:
+            walk(v, 0.0, 0)
+    sums.sort()
+    assert len(sums) == len(allp)
+    tol = np.array([2e-4 * h + 1e-5 for _, h in sums])
+    np.testing.assert_array_less(np.abs(np.array([x for x, _ in sums]) - allp), tol + 1e-4)
 
 
 def test_sfxt_fixtures_closed_forms(golden_dir):
