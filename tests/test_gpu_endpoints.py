@@ -54,17 +54,17 @@ def test_c1_masked_query_matches_oracle(frac):
 
 
 def test_mask_changes_with_cache_enabled():
-    """The cached SFXT is kept per endpoint subset: a query with another
-    subset recomputes it, the same subset reuses it, and an all-sinks mask
-    is the unmasked query."""
+    """The cache holds the last SFXT with its endpoint subset: a query with
+    the same subset reuses it, another subset recomputes it, and an
+    all-sinks mask is the unmasked query."""
     n, u, v, w = cpgamd.gen_preset(1)
     og = pyoracle.OracleGraph.from_edges(n, u, v, w)
     g = cpgamd.CpGen()
     g.load_edges(n, u, v, w)
     g.enable_tc_pfxt_static_cache(True)
     k = 3000
-    masks = [sink_mask(n, u, 0.05, 1), sink_mask(n, u, 0.05, 2), None,
-             sink_mask(n, u, 0.05, 1), sink_mask(n, u, 1.0, 3)]
+    masks = [sink_mask(n, u, 0.05, 1), sink_mask(n, u, 0.05, 1), sink_mask(n, u, 0.05, 2),
+             None, sink_mask(n, u, 1.0, 3)]
     wants = []
     for m in masks:
         d, s, _, _ = og.sfxt(m)
@@ -75,8 +75,9 @@ def test_mask_changes_with_cache_enabled():
         g.report_paths(k, endpoint_mask=m)
         got = g.get_slacks(k)
         assert len(got) == len(want) and pyoracle.first_bit_mismatch(want, got) == -1
-    np.testing.assert_array_equal(wants[2], wants[4])  # all sinks == no mask
-    assert g.tc_pfxt_static_cache_hits() >= 1
+    np.testing.assert_array_equal(wants[3], wants[4])  # all sinks == no mask
+    assert g.tc_pfxt_static_cache_hits() == 2  # steps 1 and 4
+    assert g.tc_pfxt_static_cache_misses() == 3
 
 
 def test_c5_graph_masked_small_k():
