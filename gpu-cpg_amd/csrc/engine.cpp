@@ -1071,6 +1071,9 @@ struct cpg_batch {
 #ifndef CPG_SERVER_NODES_PER_PATH
 #define CPG_SERVER_NODES_PER_PATH 16ull
 #endif
+#ifndef CPG_SERVER_HEADROOM_NODES
+#define CPG_SERVER_HEADROOM_NODES 65536ull
+#endif
 
 struct cpg_server {
   struct IndexSet {
@@ -1093,6 +1096,7 @@ struct cpg_server {
     int state = 0;              // 0 queued, 1 in flight, 2 done
     std::unique_ptr<IndexSet> index;
     std::vector<int> roots;     // kept roots (vertex ids, ascending)
+    uint64_t seed_n = 0;        // paths of the first-bound probe left in the graph handle's store
     MqQuery init{};             // slot state at admission
     std::vector<float> result;
   };
@@ -1131,7 +1135,7 @@ struct cpg_server {
   DevBuf<unsigned> hist, tiles, ghist;
   DevBuf<MqQuery> qs;
   DevBuf<MqCtl> mctl;
-  DevBuf<int> root_v, root_q, newidx;
+  DevBuf<int> newidx;
   DevBuf<uint8_t> sel;
   DevBuf<uint64_t> skey[2];
   DevBuf<uint32_t> sval[2];
@@ -1247,12 +1251,48 @@ struct cpg_server {
     q.roots.clear();
     for (int i = 0; i < nsrc; ++i)
       if (kp[i]) q.roots.push_back(h.g.srcs[i]);
+    // First bound: the single query's probe on the handle's own node store
+    // stores the roots (expanded) and their children up to a bound u0 under
+    // which at least k paths lie; admission copies them into the shared
+    // store, so the query's rounds start where a single query's would.  A
+    // probe that finds no bound leaves the roots alone (u0 = no bound).
+    uint32_t u0 = kKeyInf;
+    q.seed_n = 0;
+    const uint32_t klo = q.roots.empty() ? 0u : float_key(minc);
+    if (!q.roots.empty() && q.k > 0) {
+      const uint64_t cap = std::min<uint64_t>(
+          std::max<uint64_t>(uint64_t(1) << 22, 16ull * static_cast<uint64_t>(q.k) + q.roots.size() + (1u << 20)),
+          0x7fffffffull);
+      h.ensure_pfxt_buffers(cap);
+      PfxtCtl init{};
+      init.ukey = kKeyInf;
+      init.klo = klo;
+      init.shift = window_shift(klo, kKeyInf);
+      init.node_cap = cap;
+      h2d(h.pctl.p, &init, 1, h.stream);
+      h.keep.reserve(std::max(1, nsrc));
+      h2d(h.keep.p, kp.data(), nsrc, h.stream);
+      launch_roots(h.srcs.p, h.keep.p, nsrc, h.dist.p, h.distf.p, h.node_store(), h.pctl.p, h.stream);
+      WalkArgs wa{};
+      wa.vrec = h.vrec.p; wa.devinfo = h.devinfo.p; wa.devs = h.devs.p; wa.batch = h.batch.p;
+      wa.nodes = h.node_store(); wa.ckeys = h.ckeys.p; wa.ckeys_cap = cap; wa.ctl = h.pctl.p;
+      wa.max_abs_distf = h.max_abs_distf; wa.max_abs_wgt = h.g.max_abs_wgt;
+      wa.step_limit = h.step_limit();
+      const double tp = now_ms();
+      u0 = h.probe_bound(wa, q.k, kp, sd, minc, klo, false);
+      stats.probe_ms += now_ms() - tp;
+      PfxtCtl pc{};
+      d2h(&pc, h.pctl.p, 1, h.stream);
+      CPG_HIP(hipStreamSynchronize(h.stream));
+      if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
+      q.seed_n = std::min<uint64_t>(pc.node_n, cap);
+    }
     h.vrec.swap(ix->vrec); h.devs.swap(ix->devs); h.devinfo.swap(ix->devinfo); h.distf.swap(ix->distf);
     h.sfxt_valid = false;  // the handle's index went to the query
     MqQuery m{};
-    m.ukey = kKeyInf;
-    m.klo = q.roots.empty() ? 0u : float_key(minc);
-    m.shift = window_shift_bins(m.klo, kKeyInf, kMqBins);
+    m.ukey = u0;
+    m.klo = klo;
+    m.shift = window_shift_bins(m.klo, u0, kMqBins);
     m.k = static_cast<unsigned long long>(q.k);
     m.batch_target = std::max<unsigned long long>(256, m.k / 8);
     m.done = (q.roots.empty() || q.k == 0) ? 1 : 0;
@@ -1261,7 +1301,7 @@ struct cpg_server {
     m.devs = ix->devs.p;
     m.mag = 2.0f * h.max_abs_distf + h.g.max_abs_wgt + 1.0f;
     m.step_limit = h.step_limit();
-    m.all = 1;
+    m.all = u0 == kKeyInf ? 1 : 0;  // without a bound the first round expands every root
     q.init = m;
     q.index = std::move(ix);
   }
@@ -1306,14 +1346,14 @@ struct cpg_server {
         if (dead_slots.empty()) break;
         compact(st[cur].cap);
       }
-      const uint64_t need = CPG_SERVER_NODES_PER_PATH * static_cast<uint64_t>(q.k) +
-                            G.h->g.srcs.size() + 65536;
+      queue.pop_front();
+      if (!q.index) prepare(q);
+      const uint64_t need = CPG_SERVER_NODES_PER_PATH * static_cast<uint64_t>(q.k) + q.seed_n +
+                            CPG_SERVER_HEADROOM_NODES;
       if (node_n + need > st[cur].cap) {
         compact(std::max(st[cur].cap, node_n + 2 * need));
         if (node_n + need > st[cur].cap) compact(2 * (node_n + need));
       }
-      queue.pop_front();
-      if (!q.index) prepare(q);
       const int slot = free_slots.back();
       free_slots.pop_back();
       q.slot = slot;
@@ -1324,15 +1364,11 @@ struct cpg_server {
       stats.peak_index_bytes = std::max(stats.peak_index_bytes, index_bytes);
       ++in_flight;
       h2d(qs.p + slot, &q.init, 1, stream);
-      const int nr = static_cast<int>(q.roots.size());
-      if (nr > 0) {
-        root_v.reserve(nr);
-        root_q.reserve(nr);
-        std::vector<int> rq(nr, slot);
-        h2d(root_v.p, q.roots.data(), nr, stream);
-        h2d(root_q.p, rq.data(), nr, stream);
-        launch_mq_roots(args(q.index->distf.p), root_v.p, root_q.p, nr, stream);
-        node_n += static_cast<uint64_t>(nr);
+      if (q.seed_n > 0) {
+        // the probe's paths are still in the graph handle's node store
+        launch_mq_seed(G.h->node_store(), q.seed_n, st[cur].view(), st[cur].nq.p, node_n, slot,
+                       mctl.p, grid, stream);
+        node_n += q.seed_n;
       }
       CPG_HIP(hipStreamSynchronize(stream));
       stats.admitted++;
@@ -1399,6 +1435,7 @@ struct cpg_server {
       q.slot = -1;
       back.push_back(q.id);
       index_bytes -= graphs[q.graph].index_bytes;
+      graphs[q.graph].pool.push_back(std::move(q.index));  // prepared again on re-admission
       slot_query[sl] = -1;
     }
     for (auto it = back.rbegin(); it != back.rend(); ++it) queue.push_front(*it);

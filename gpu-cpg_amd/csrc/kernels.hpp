@@ -193,6 +193,10 @@ void launch_zero_u32(unsigned* p, uint64_t n, hipStream_t s);
 // into dst with newidx[old] = new slot (-1 for a dead node) and the
 // parents remapped.
 uint64_t mq_compact_tiles(uint64_t n);
+// Appends src[0..cnt) at dst[base..) as query slot q and sets node_n to
+// base + cnt (no fill pass may run concurrently).
+void launch_mq_seed(const NodeStore& src, uint64_t cnt, const NodeStore& dst, int* dst_nq,
+                    uint64_t base, int q, MqCtl* ctl, int grid, hipStream_t s);
 void launch_mq_count_live(const MqArgs& a, uint64_t n, unsigned* tile_cnt, hipStream_t s);
 void launch_mq_compact(const MqArgs& a, uint64_t n, const NodeStore& dst, int* dst_nq,
                        const unsigned* tile_off, int* newidx, uint64_t n_live, int grid,

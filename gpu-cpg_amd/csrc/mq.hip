@@ -352,6 +352,27 @@ __global__ void __launch_bounds__(kMqBlock) mq_compact_kernel(MqArgs a, uint64_t
   }
 }
 
+// Appends a single-query node store's first cnt paths (a bound probe's
+// roots and children) at dst[base..) for query slot q, parents shifted.
+__global__ void mq_seed_kernel(NodeStore src, uint64_t cnt, NodeStore dst, int* dst_nq,
+                               uint64_t base, int q) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < cnt;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t d = base + i;
+    const int p = src.parent[i];
+    dst.cost[d] = src.cost[i];
+    dst.to[d] = src.to[i];
+    dst.from[d] = src.from[i];
+    dst.parent[d] = p < 0 ? -1 : static_cast<int>(base + static_cast<uint64_t>(p));
+    dst.level[d] = src.level[i];
+    dst.key[d] = src.key[i];
+    dst.flag[d] = src.flag[i];
+    dst_nq[d] = q;
+  }
+}
+
+__global__ void mq_set_node_n_kernel(MqCtl* ctl, unsigned long long n) { ctl->node_n = n; }
+
 __global__ void mq_remap_parents_kernel(int* parent, const int* newidx, uint64_t n) {
   for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
        j += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -425,6 +446,17 @@ void launch_mq_build_keys(const MqArgs& a, const uint8_t* sel, uint64_t nmax, ui
 }
 
 uint64_t mq_compact_tiles(uint64_t n) { return (n + kCompactTile - 1) / kCompactTile; }
+
+void launch_mq_seed(const NodeStore& src, uint64_t cnt, const NodeStore& dst, int* dst_nq,
+                    uint64_t base, int q, MqCtl* ctl, int grid, hipStream_t s) {
+  if (cnt > 0) {
+    hipLaunchKernelGGL(mq_seed_kernel, dim3(grid), dim3(kMqBlock), 0, s, src, cnt, dst, dst_nq, base, q);
+    CPG_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(mq_set_node_n_kernel, dim3(1), dim3(1), 0, s, ctl,
+                     static_cast<unsigned long long>(base + cnt));
+  CPG_HIP(hipGetLastError());
+}
 
 void launch_mq_count_live(const MqArgs& a, uint64_t n, unsigned* tile_cnt, hipStream_t s) {
   const uint64_t nt = mq_compact_tiles(n);
