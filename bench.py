@@ -18,6 +18,17 @@ Prints one JSON line.  Besides the contract fields it carries
   cpu_baseline    the oracle's host SFXT + best-first PFXT on this host
                   (single thread), same graph and K
   exact           GPU costs equal the CPU baseline's bit for bit
+  batched         256 seeded designs of uneven size (5K-200K vertices), one
+                  query each (K 100-100K): the batched engine (one union
+                  SFXT, multi-query rounds replayed from a captured HIP
+                  graph) against a loop of plain single queries on resident
+                  graphs; queries/s, speedup, exactness
+  served          8 resident graphs (200K-2.6M vertices, up to 70M edges),
+                  a seeded stream of queries each with an endpoint subset
+                  (1-100% of sinks) and K log-uniform 1K-1M: saturated
+                  queries/s of the served engine and of plain queries one
+                  at a time, then a Poisson arrival replay at 0.8 x plain
+                  capacity with p50/p99 latency of both; exactness
 """
 import argparse
 import json
@@ -62,6 +73,146 @@ def load_golden_counts():
         return None
 
 
+def batched_section(np, cpgamd, reps=3):
+    """256 designs: batched engine vs a loop of plain single queries."""
+    t0 = time.time()
+    designs = [cpgamd.gen_preset(4, i) + (int(cpgamd.preset_k(4, i)),) for i in range(256)]
+    log("batched: generated 256 designs in %.1fs" % (time.time() - t0))
+    b = cpgamd.Batch()
+    for n, u, v, w, k in designs:
+        b.add(n, u, v, w, k)
+    batch_ms = []
+    for rep in range(reps):
+        s0 = time.perf_counter()
+        b.run()
+        batch_ms.append((time.perf_counter() - s0) * 1e3)
+        log("batched run %d: %.1f ms" % (rep, batch_ms[-1]))
+    hs = []
+    for n, u, v, w, k in designs:
+        g = cpgamd.CpGen()
+        g.load_edges(n, u, v, w)
+        hs.append(g)
+    for g, d in zip(hs, designs):  # first query uploads the graph and sizes buffers
+        g.reset()
+        g.report_paths(d[4])
+    s0 = time.perf_counter()
+    for g, d in zip(hs, designs):
+        g.reset()
+        g.report_paths(d[4])
+    plain_ms = (time.perf_counter() - s0) * 1e3
+    exact = sum(1 for q, (g, d) in enumerate(zip(hs, designs))
+                if np.array_equal(g.get_slacks(d[4]).view(np.uint32),
+                                  b.get_slacks(q, d[4]).view(np.uint32)))
+    best = min(batch_ms[1:]) if len(batch_ms) > 1 else batch_ms[0]
+    return {"workload": "256 designs, 5K-200K vertices each (13.5M total), K 100-100K (3.73M total)",
+            "batch_ms": round(best, 3), "queries_per_s": round(256 / (best * 1e-3), 1),
+            "plain_loop_ms": round(plain_ms, 3),
+            "plain_queries_per_s": round(256 / (plain_ms * 1e-3), 1),
+            "speedup": round(plain_ms / best, 3), "exact": "%d/256" % exact,
+            "batch_latency_ms": round(best, 3)}
+
+
+def served_section(np, cpgamd, nq=48, load=0.8):
+    """Served stream of endpoint-subset queries on 8 resident graphs."""
+    t0 = time.time()
+    graphs = [cpgamd.gen_preset(5, i) for i in range(8)]
+    log("served: generated 8 graphs in %.1fs" % (time.time() - t0))
+    rng = np.random.default_rng(55)
+    sinks = [np.nonzero(np.bincount(u, minlength=n) == 0)[0] for n, u, v, w in graphs]
+    queries = []
+    for _ in range(nq):
+        gi = int(rng.integers(0, 8))
+        frac = max(0.01, float(1.0 - rng.random()))  # (0, 1]
+        k = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))))
+        n = graphs[gi][0]
+        m = np.zeros(n, np.uint8)
+        m[rng.choice(sinks[gi], max(1, int(np.ceil(frac * len(sinks[gi])))), replace=False)] = 1
+        queries.append((gi, k, m))
+    gaps = rng.exponential(1.0, size=nq)  # Poisson arrivals, scaled to the rate below
+    hs = []
+    for n, u, v, w in graphs:
+        g = cpgamd.CpGen()
+        g.load_edges(n, u, v, w)
+        g.report_paths(1000)
+        hs.append(g)
+    plain_res = []
+    s0 = time.perf_counter()
+    for gi, k, m in queries:
+        hs[gi].reset()
+        hs[gi].report_paths(k, endpoint_mask=m)
+        plain_res.append(hs[gi].get_slacks(k))
+    plain_ms = (time.perf_counter() - s0) * 1e3
+    log("served: plain saturated %.1f ms" % plain_ms)
+    S = cpgamd.Server()
+    for n, u, v, w in graphs:
+        S.add_graph(n, u, v, w)
+
+    def drain(ids):
+        while S.step() > 0:
+            pass
+        S.poll()
+        good = sum(1 for q, r in zip(ids, plain_res)
+                   if np.array_equal(S.get_slacks(q, len(r)).view(np.uint32), r.view(np.uint32)))
+        for q in ids:
+            S.release(q)
+        return good
+
+    drain([S.submit(gi, k, m) for gi, k, m in queries])  # first run sizes the store
+    s0 = time.perf_counter()
+    exact = drain([S.submit(gi, k, m) for gi, k, m in queries])
+    served_ms = (time.perf_counter() - s0) * 1e3
+    log("served: saturated %.1f ms, exact %d/%d" % (served_ms, exact, nq))
+    rate = load * nq / (plain_ms * 1e-3)  # queries/s offered
+    arrive = np.cumsum(gaps) / rate * 1e3  # ms after the start
+
+    now = cpgamd.now_ms
+    # plain replay: one query at a time in arrival order
+    lat_plain = []
+    start = now()
+    for i, (gi, k, m) in enumerate(queries):
+        t_arr = start + arrive[i]
+        while now() < t_arr:
+            time.sleep(max(0.0, min(0.002, (t_arr - now()) * 1e-3)))
+        hs[gi].reset()
+        hs[gi].report_paths(k, endpoint_mask=m)
+        lat_plain.append(now() - t_arr)
+    plain_span = now() - start
+    # served replay: submit on arrival, step while work is queued or in flight
+    lat_served = {}
+    ids = {}
+    start = now()
+    nxt = 0
+    while len(lat_served) < nq:
+        t = now()
+        while nxt < nq and start + arrive[nxt] <= t:
+            gi, k, m = queries[nxt]
+            ids[S.submit(gi, k, m)] = start + arrive[nxt]
+            nxt += 1
+        busy = S.step()
+        for q in S.poll():
+            lat_served[q] = S.times(q)[2] - ids[q]
+            S.release(q)
+        if busy == 0 and nxt < nq:  # idle until the next arrival
+            time.sleep(max(0.0, min(0.002, (start + arrive[nxt] - now()) * 1e-3)))
+    served_span = now() - start
+    lp = np.array(lat_plain)
+    ls = np.array(list(lat_served.values()))
+    return {"workload": "8 graphs (200K-2.6M vertices, 4M-70M edges), %d queries, endpoint subsets "
+                        "1-100%% of sinks, K 1K-1M log-uniform" % nq,
+            "saturated": {"served_ms": round(served_ms, 2), "plain_ms": round(plain_ms, 2),
+                          "served_queries_per_s": round(nq / (served_ms * 1e-3), 2),
+                          "plain_queries_per_s": round(nq / (plain_ms * 1e-3), 2),
+                          "speedup": round(plain_ms / served_ms, 3)},
+            "poisson": {"offered_queries_per_s": round(rate, 2), "load_vs_plain_capacity": load,
+                        "served": {"p50_ms": round(float(np.percentile(ls, 50)), 2),
+                                   "p99_ms": round(float(np.percentile(ls, 99)), 2),
+                                   "queries_per_s": round(nq / (served_span * 1e-3), 2)},
+                        "plain": {"p50_ms": round(float(np.percentile(lp, 50)), 2),
+                                  "p99_ms": round(float(np.percentile(lp, 99)), 2),
+                                  "queries_per_s": round(nq / (plain_span * 1e-3), 2)}},
+            "exact": "%d/%d" % (exact, nq), "server": S.stats()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +221,9 @@ def main():
     ap.add_argument("--k", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-trials", type=int, default=2)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-queries", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported"
@@ -200,6 +354,11 @@ def main():
             "nproc": os.cpu_count()}
         out["exact"] = bool(len(cpu_costs) == len(costs)
                             and pyoracle.first_bit_mismatch(cpu_costs, costs) == -1)
+    del g
+    if not args.no_batched:
+        out["batched"] = batched_section(np, cpgamd)
+    if not args.no_served:
+        out["served"] = served_section(np, cpgamd, args.served_queries)
     print(json.dumps(out), flush=True)
 
 
