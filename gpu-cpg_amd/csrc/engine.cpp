@@ -1079,12 +1079,19 @@ struct cpg_server {
   struct IndexSet {
     DevBuf<int4> vrec, devs;
     DevBuf<int2> devinfo;
-    DevBuf<float> distf;
   };
   struct Graph {
     std::unique_ptr<cpg_handle> h;
     std::vector<std::unique_ptr<IndexSet>> pool;  // index buffers of finished queries
     int64_t index_bytes = 0;
+    struct Multi {  // the multi-subset SFXT of an admission group
+      DevBuf<unsigned long long> best;
+      DevBuf<int> dist, succ;
+      DevBuf<float> distf;
+      DevBuf<uint8_t> masks;
+      DevBuf<unsigned> maxd;
+      DevBuf<SfxtCtl> ctls;
+    } mm;
   };
   struct Query {
     int64_t id = 0;
@@ -1114,7 +1121,7 @@ struct cpg_server {
     NodeStore view() { return NodeStore{cost.p, to.p, from.p, parent.p, level.p, key.p, flag.p, cap}; }
   };
 
-  std::vector<Graph> graphs;
+  std::vector<std::unique_ptr<Graph>> graphs;
   std::deque<int64_t> queue;  // submitted, not yet admitted
   std::unordered_map<int64_t, std::unique_ptr<Query>> queries;
   std::vector<int64_t> slot_query;  // query id per slot (-1 free)
@@ -1181,12 +1188,12 @@ struct cpg_server {
   }
 
   int add_graph(const EdgeList& el) {
-    Graph G;
-    G.h = std::make_unique<cpg_handle>();
-    G.h->set_graph(el);
-    G.h->upload_graph();
-    const int64_t n = G.h->g.n, m = G.h->g.m;
-    G.index_bytes = 28 * n + 16 * m;  // vrec 16, devinfo 8, distf 4 per vertex; devs 16 per edge
+    auto G = std::make_unique<Graph>();
+    G->h = std::make_unique<cpg_handle>();
+    G->h->set_graph(el);
+    G->h->upload_graph();
+    const int64_t n = G->h->g.n, m = G->h->g.m;
+    G->index_bytes = 24 * n + 16 * m;  // vrec 16, devinfo 8 per vertex; devs 16 per edge
     graphs.push_back(std::move(G));
     return static_cast<int>(graphs.size()) - 1;
   }
@@ -1198,7 +1205,7 @@ struct cpg_server {
     q->id = next_id++;
     q->graph = graph;
     q->k = k;
-    if (mask) q->mask.assign(mask, mask + graphs[graph].h->g.n);
+    if (mask) q->mask.assign(mask, mask + graphs[graph]->h->g.n);
     q->t_submit = now_ms();
     const int64_t id = q->id;
     queries[id] = std::move(q);
@@ -1220,48 +1227,31 @@ struct cpg_server {
     return a;
   }
 
-  // The query's SFXT for its endpoint subset on its graph's handle, its
-  // roots, and the index buffers taken over from the handle.
-  void prepare(Query& q) {
-    Graph& G = graphs[q.graph];
-    cpg_handle& h = *G.h;
-    std::unique_ptr<IndexSet> ix;
-    if (!G.pool.empty()) {
-      ix = std::move(G.pool.back());
-      G.pool.pop_back();
-    } else {
-      ix = std::make_unique<IndexSet>();
-    }
-    h.vrec.swap(ix->vrec); h.devs.swap(ix->devs); h.devinfo.swap(ix->devinfo); h.distf.swap(ix->distf);
-    h.cache_enabled = false;
-    h.sfxt_valid = false;
-    const double t0 = now_ms();
-    h.run_sfxt(false, q.mask.empty() ? nullptr : q.mask.data());
-    stats.sfxt_ms += now_ms() - t0;
+  // The first bound of query q, whose SFXT columns and index are in place:
+  // its roots, then the single query's probe on the graph handle's own node
+  // store, which stores the roots (expanded) and their children up to a
+  // bound u0 under which at least k paths lie (admission copies them into
+  // the shared store, so the query's rounds start where a single query's
+  // would).  A probe that finds no bound leaves the roots alone (u0 = no
+  // bound, the first round expands them all).
+  void probe_query(Query& q, cpg_handle& h, const SfxtArgs& aq, float max_abs_distf) {
     const int nsrc = static_cast<int>(h.g.srcs.size());
     std::vector<int> sd(nsrc);
     h.src_dist.reserve(std::max(1, nsrc));
-    launch_gather_int(h.dist.p, h.srcs.p, nsrc, h.src_dist.p, h.stream);
+    launch_gather_int(aq.dist, h.srcs.p, nsrc, h.src_dist.p, h.stream);
     d2h(sd.data(), h.src_dist.p, nsrc, h.stream);
     CPG_HIP(hipStreamSynchronize(h.stream));
     std::vector<uint8_t> kp;
     int64_t nroots = 0, dropped = 0;
     float minc = 0.0f;
     pick_roots(sd, true, 0.005f, kp, nroots, dropped, minc);
-    q.roots.clear();
-    for (int i = 0; i < nsrc; ++i)
-      if (kp[i]) q.roots.push_back(h.g.srcs[i]);
-    // First bound: the single query's probe on the handle's own node store
-    // stores the roots (expanded) and their children up to a bound u0 under
-    // which at least k paths lie; admission copies them into the shared
-    // store, so the query's rounds start where a single query's would.  A
-    // probe that finds no bound leaves the roots alone (u0 = no bound).
+    const int64_t nkept = nroots - dropped;
+    const uint32_t klo = nkept > 0 ? float_key(minc) : 0u;
     uint32_t u0 = kKeyInf;
     q.seed_n = 0;
-    const uint32_t klo = q.roots.empty() ? 0u : float_key(minc);
-    if (!q.roots.empty() && q.k > 0) {
+    if (nkept > 0 && q.k > 0) {
       const uint64_t cap = std::min<uint64_t>(
-          std::max<uint64_t>(uint64_t(1) << 22, 16ull * static_cast<uint64_t>(q.k) + q.roots.size() + (1u << 20)),
+          std::max<uint64_t>(uint64_t(1) << 22, 16ull * static_cast<uint64_t>(q.k) + nkept + (1u << 20)),
           0x7fffffffull);
       h.ensure_pfxt_buffers(cap);
       PfxtCtl init{};
@@ -1272,11 +1262,11 @@ struct cpg_server {
       h2d(h.pctl.p, &init, 1, h.stream);
       h.keep.reserve(std::max(1, nsrc));
       h2d(h.keep.p, kp.data(), nsrc, h.stream);
-      launch_roots(h.srcs.p, h.keep.p, nsrc, h.dist.p, h.distf.p, h.node_store(), h.pctl.p, h.stream);
+      launch_roots(h.srcs.p, h.keep.p, nsrc, aq.dist, aq.distf, h.node_store(), h.pctl.p, h.stream);
       WalkArgs wa{};
-      wa.vrec = h.vrec.p; wa.devinfo = h.devinfo.p; wa.devs = h.devs.p; wa.batch = h.batch.p;
+      wa.vrec = aq.vrec; wa.devinfo = aq.devinfo; wa.devs = aq.devs; wa.batch = h.batch.p;
       wa.nodes = h.node_store(); wa.ckeys = h.ckeys.p; wa.ckeys_cap = cap; wa.ctl = h.pctl.p;
-      wa.max_abs_distf = h.max_abs_distf; wa.max_abs_wgt = h.g.max_abs_wgt;
+      wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = h.g.max_abs_wgt;
       wa.step_limit = h.step_limit();
       const double tp = now_ms();
       u0 = h.probe_bound(wa, q.k, kp, sd, minc, klo, false);
@@ -1287,23 +1277,143 @@ struct cpg_server {
       if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
       q.seed_n = std::min<uint64_t>(pc.node_n, cap);
     }
-    h.vrec.swap(ix->vrec); h.devs.swap(ix->devs); h.devinfo.swap(ix->devinfo); h.distf.swap(ix->distf);
-    h.sfxt_valid = false;  // the handle's index went to the query
     MqQuery m{};
     m.ukey = u0;
     m.klo = klo;
     m.shift = window_shift_bins(m.klo, u0, kMqBins);
     m.k = static_cast<unsigned long long>(q.k);
     m.batch_target = std::max<unsigned long long>(256, m.k / 8);
-    m.done = (q.roots.empty() || q.k == 0) ? 1 : 0;
-    m.vrec = ix->vrec.p;
-    m.devinfo = ix->devinfo.p;
-    m.devs = ix->devs.p;
-    m.mag = 2.0f * h.max_abs_distf + h.g.max_abs_wgt + 1.0f;
+    m.done = (nkept == 0 || q.k == 0) ? 1 : 0;
+    m.vrec = aq.vrec;
+    m.devinfo = aq.devinfo;
+    m.devs = aq.devs;
+    m.mag = 2.0f * max_abs_distf + h.g.max_abs_wgt + 1.0f;
     m.step_limit = h.step_limit();
-    m.all = u0 == kKeyInf ? 1 : 0;  // without a bound the first round expands every root
+    m.all = u0 == kKeyInf ? 1 : 0;
     q.init = m;
-    q.index = std::move(ix);
+  }
+
+  // Takes a free slot for q and copies its probe's paths (still in the
+  // graph handle's node store) into the shared store.
+  void admit_one(Query& q) {
+    Graph& G = *graphs[q.graph];
+    const uint64_t need = CPG_SERVER_NODES_PER_PATH * static_cast<uint64_t>(q.k) + q.seed_n +
+                          CPG_SERVER_HEADROOM_NODES;
+    if (node_n + need > st[cur].cap) {
+      compact(std::max(st[cur].cap, node_n + 2 * need));
+      if (node_n + need > st[cur].cap) compact(2 * (node_n + need));
+    }
+    const int slot = free_slots.back();
+    free_slots.pop_back();
+    q.slot = slot;
+    q.state = 1;
+    q.t_admit = now_ms();
+    slot_query[slot] = q.id;
+    index_bytes += G.index_bytes;
+    stats.peak_index_bytes = std::max(stats.peak_index_bytes, index_bytes);
+    ++in_flight;
+    h2d(qs.p + slot, &q.init, 1, stream);
+    if (q.seed_n > 0)
+      launch_mq_seed(G.h->node_store(), q.seed_n, st[cur].view(), st[cur].nq.p, node_n, slot,
+                     mctl.p, grid, stream);
+    node_n += q.seed_n;
+    CPG_HIP(hipStreamSynchronize(stream));
+    stats.admitted++;
+  }
+
+  // Admits a group of queued queries on one graph: one level loop relaxes
+  // all their endpoint subsets (padded to a power of two with empty ones),
+  // then each query gets its deviation index, its first bound and a slot.
+  void admit_group(int gi, const std::vector<int64_t>& ids) {
+    Graph& G = *graphs[gi];
+    cpg_handle& h = *G.h;
+    const size_t n = static_cast<size_t>(h.g.n), m = static_cast<size_t>(h.g.m);
+    int Q = 1;
+    while (Q < static_cast<int>(ids.size())) Q <<= 1;
+    h.deps.reserve(n); h.queue.reserve(n); h.lvlp.reserve(n + 2); h.level.reserve(n);
+    h.big.reserve(n); h.sctl.reserve(1);
+    if (h.g.max_out_degree > kSfxtLdsSort) h.devs_tmp.reserve(m);
+    Graph::Multi& M = G.mm;
+    M.best.reserve(n * Q); M.dist.reserve(n * Q); M.succ.reserve(n * Q); M.distf.reserve(n * Q);
+    M.masks.reserve(n * Q); M.maxd.reserve(Q); M.ctls.reserve(Q);
+    std::vector<uint8_t> hm(n * Q, 0);
+    for (size_t q = 0; q < ids.size(); ++q) {
+      const Query& qq = *queries[ids[q]];
+      if (qq.mask.empty()) std::fill(hm.begin() + q * n, hm.begin() + (q + 1) * n, uint8_t(1));
+      else std::copy(qq.mask.begin(), qq.mask.end(), hm.begin() + q * n);
+    }
+    const double t0 = now_ms();
+    h2d(M.masks.p, hm.data(), n * Q, h.stream);
+    CPG_HIP(hipMemsetAsync(h.sctl.p, 0, sizeof(SfxtCtl), h.stream));
+    CPG_HIP(hipMemsetAsync(M.ctls.p, 0, sizeof(SfxtCtl) * Q, h.stream));
+    SfxtArgs a{};
+    a.n = h.g.n;
+    a.nsinks = static_cast<int>(h.g.sinks.size());
+    a.oadjp = h.oadjp.p; a.oadj = h.oadj.p; a.owgt = h.owgt.p;
+    a.iadjp = h.iadjp.p; a.iedge = h.iedge.p; a.sinks = h.sinks.p;
+    a.deps = h.deps.p; a.queue = h.queue.p; a.lvlp = h.lvlp.p; a.level = h.level.p;
+    a.big = h.big.p;
+    a.devs_tmp = h.g.max_out_degree > kSfxtLdsSort ? h.devs_tmp.p : nullptr;
+    a.ctl = h.sctl.p;
+    SfxtMulti mm{};
+    mm.q = Q;
+    mm.masks = M.masks.p; mm.best = M.best.p; mm.dist = M.dist.p; mm.succ = M.succ.p;
+    mm.distf = M.distf.p; mm.maxd = M.maxd.p;
+    SfxtCtl c{};
+    std::vector<unsigned> maxd(Q, 0u);
+    if (n > 0) {
+      launch_sfxt_init_multi(a, mm, h.stream);
+      // one wave per vertex: about one block per 8 vertices of an average level
+      const int64_t per_level = h.last_diameter > 0 ? static_cast<int64_t>(n) / h.last_diameter : 4096;
+      const int blocks = static_cast<int>(std::min<int64_t>(128, std::max<int64_t>(16, per_level / 8)));
+      launch_sfxt_levels_multi(a, mm, blocks, h.stream);
+      d2h(&c, h.sctl.p, 1, h.stream);
+      d2h(maxd.data(), M.maxd.p, Q, h.stream);
+      CPG_HIP(hipStreamSynchronize(h.stream));
+      if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
+    }
+    h.diameter = c.num_levels;
+    h.last_diameter = c.num_levels;
+    h.sfxt_valid = false;  // the handle's own SFXT buffers were not written
+    const int lv = sfxt_lift_levels(c.num_levels);
+    h.lift.reserve(static_cast<size_t>(lv) * std::max<size_t>(n, 1));
+    stats.sfxt_ms += now_ms() - t0;
+    stats.sfxt_groups++;
+    for (size_t q = 0; q < ids.size(); ++q) {
+      Query& qq = *queries[ids[q]];
+      std::unique_ptr<IndexSet> ix;
+      if (!G.pool.empty()) {
+        ix = std::move(G.pool.back());
+        G.pool.pop_back();
+      } else {
+        ix = std::make_unique<IndexSet>();
+      }
+      ix->vrec.reserve(std::max<size_t>(n, 1));
+      ix->devinfo.reserve(std::max<size_t>(n, 1));
+      ix->devs.reserve(std::max<size_t>(m, 1));
+      SfxtArgs aq = a;
+      aq.dist = M.dist.p + q * n;
+      aq.succ = M.succ.p + q * n;
+      aq.distf = M.distf.p + q * n;
+      aq.vrec = ix->vrec.p; aq.devinfo = ix->devinfo.p; aq.devs = ix->devs.p;
+      aq.lift = h.lift.p;
+      aq.ctl = M.ctls.p + q;
+      float mad = 0.0f;
+      std::memcpy(&mad, &maxd[q], 4);
+      if (n > 0) {
+        const double ti = now_ms();
+        launch_sfxt_index(aq, lv, h.stream);
+        SfxtCtl cq{};
+        d2h(&cq, aq.ctl, 1, h.stream);
+        CPG_HIP(hipStreamSynchronize(h.stream));
+        if (cq.nsv_err == 2) throw std::runtime_error("SFXT long deviation list without sort buffer");
+        if (cq.nsv_err) throw std::runtime_error("SFXT next-smaller descent found no answer");
+        stats.sfxt_ms += now_ms() - ti;
+      }
+      probe_query(qq, h, aq, mad);
+      qq.index = std::move(ix);
+      admit_one(qq);
+    }
   }
 
   // Copies the live nodes (slots not done) into the other buffer set, of
@@ -1337,41 +1447,30 @@ struct cpg_server {
     stats.compact_ms += now_ms() - t0;
   }
 
+  // FIFO admission between rounds.  The head query's graph forms a group
+  // with the next queued queries on the same graph (at most 16), as long as
+  // slots are free and the in-flight indexes stay within the budget (the
+  // first query is admitted whatever its size).
   void admit() {
     while (!queue.empty()) {
-      Query& q = *queries[queue.front()];
-      Graph& G = graphs[q.graph];
-      if (in_flight > 0 && index_bytes + G.index_bytes > budget) break;  // HBM admission
       if (free_slots.empty()) {
         if (dead_slots.empty()) break;
         compact(st[cur].cap);
       }
-      queue.pop_front();
-      if (!q.index) prepare(q);
-      const uint64_t need = CPG_SERVER_NODES_PER_PATH * static_cast<uint64_t>(q.k) + q.seed_n +
-                            CPG_SERVER_HEADROOM_NODES;
-      if (node_n + need > st[cur].cap) {
-        compact(std::max(st[cur].cap, node_n + 2 * need));
-        if (node_n + need > st[cur].cap) compact(2 * (node_n + need));
+      const int gi = queries[queue.front()]->graph;
+      const int64_t gb = graphs[gi]->index_bytes;
+      std::vector<int64_t> ids;
+      int64_t bytes = index_bytes;
+      for (int64_t id : queue) {
+        if (static_cast<int>(ids.size()) >= std::min<int>(16, static_cast<int>(free_slots.size()))) break;
+        if (queries[id]->graph != gi) continue;
+        if ((in_flight > 0 || !ids.empty()) && bytes + gb > budget) break;  // HBM admission
+        ids.push_back(id);
+        bytes += gb;
       }
-      const int slot = free_slots.back();
-      free_slots.pop_back();
-      q.slot = slot;
-      q.state = 1;
-      q.t_admit = now_ms();
-      slot_query[slot] = q.id;
-      index_bytes += G.index_bytes;
-      stats.peak_index_bytes = std::max(stats.peak_index_bytes, index_bytes);
-      ++in_flight;
-      h2d(qs.p + slot, &q.init, 1, stream);
-      if (q.seed_n > 0) {
-        // the probe's paths are still in the graph handle's node store
-        launch_mq_seed(G.h->node_store(), q.seed_n, st[cur].view(), st[cur].nq.p, node_n, slot,
-                       mctl.p, grid, stream);
-        node_n += q.seed_n;
-      }
-      CPG_HIP(hipStreamSynchronize(stream));
-      stats.admitted++;
+      if (ids.empty()) break;
+      for (int64_t id : ids) queue.erase(std::find(queue.begin(), queue.end(), id));
+      admit_group(gi, ids);
     }
   }
 
@@ -1413,7 +1512,7 @@ struct cpg_server {
       q.state = 2;
       q.t_done = t1;
       completed.push_back(q.id);
-      Graph& G = graphs[q.graph];
+      Graph& G = *graphs[q.graph];
       G.pool.push_back(std::move(q.index));
       index_bytes -= G.index_bytes;
       --in_flight;
@@ -1434,8 +1533,8 @@ struct cpg_server {
       q.state = 0;
       q.slot = -1;
       back.push_back(q.id);
-      index_bytes -= graphs[q.graph].index_bytes;
-      graphs[q.graph].pool.push_back(std::move(q.index));  // prepared again on re-admission
+      index_bytes -= graphs[q.graph]->index_bytes;
+      graphs[q.graph]->pool.push_back(std::move(q.index));  // prepared again on re-admission
       slot_query[sl] = -1;
     }
     for (auto it = back.rbegin(); it != back.rend(); ++it) queue.push_front(*it);

@@ -49,8 +49,25 @@ struct SfxtArgs {
   SfxtCtl* ctl;
 };
 
+// Q endpoint subsets relaxed in one level loop (served queries): the
+// levels, queue and deps of SfxtArgs are shared; per subset q its sink
+// mask masks[q*n..], distances dist/succ/distf[q*n..] and max |distf|
+// bits maxd[q]; candidates best[v*Q + q].  Q is a power of two <= 64.
+struct SfxtMulti {
+  int q;
+  int pad;
+  const uint8_t* masks;       // Q x n bytes, nonzero = chosen sink
+  unsigned long long* best;   // n x Q
+  int* dist;                  // Q x n
+  int* succ;                  // Q x n
+  float* distf;               // Q x n
+  unsigned* maxd;             // Q
+};
+
 // Number of lift levels for a graph of num_levels levels (2^levels > chain length + 1).
 int sfxt_lift_levels(int num_levels);
+void launch_sfxt_init_multi(const SfxtArgs& args, const SfxtMulti& m, hipStream_t stream);
+int launch_sfxt_levels_multi(const SfxtArgs& args, const SfxtMulti& m, int blocks, hipStream_t stream);
 // Per-vertex init, then the cooperative level loop (returns the grid used).
 void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);

@@ -251,6 +251,123 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
   }
 }
 
+// Per-vertex state before the multi-subset level loop.
+__global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMulti m) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gthreads = gridDim.x * blockDim.x;
+  const size_t n = static_cast<size_t>(a.n);
+  for (int v = gtid; v < a.n; v += gthreads) {
+    a.deps[v] = a.oadjp[v + 1] - a.oadjp[v];
+    a.level[v] = -1;
+    for (int q = 0; q < m.q; ++q) {
+      m.best[static_cast<size_t>(v) * m.q + q] = ~0ull;
+      m.dist[q * n + v] = kIntMax;
+      m.succ[q * n + v] = -1;
+      m.distf[q * n + v] = static_cast<float>(kIntMax) / static_cast<float>(kScaleUp);
+    }
+  }
+  for (int i = gtid; i < a.nsinks; i += gthreads) a.queue[i] = a.sinks[i];
+  for (int q = gtid; q < m.q; q += gthreads) m.maxd[q] = 0u;
+  if (gtid == 0) a.ctl->qtail = static_cast<unsigned>(a.nsinks);
+}
+
+// The level loop for Q endpoint subsets at once.  Levels, the queue and
+// the fanin release are the single loop's; one wave takes a vertex: lanes
+// q < Q settle its distance in subset q from best[u*Q + q], then the wave
+// walks the fanin records 64/Q at a time with lane l on record l / Q and
+// subset l % Q, offering that subset's candidate to the record's fanin;
+// the subset-0 lane of each record releases it.  The per-level grid
+// barrier is shared by all Q subsets, so Q subsets cost about one loop.
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_multi_kernel(SfxtArgs a, SfxtMulti m) {
+  __shared__ int s_q[kLdsQueue];
+  __shared__ int s_qn;
+  __shared__ unsigned s_base;
+  SfxtCtl* ctl = a.ctl;
+  const unsigned nblocks = gridDim.x;
+  const int lane = __lane_id();
+  const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwaves = static_cast<int>((gridDim.x * blockDim.x) >> 6);
+  const int Q = m.q;
+  const int R = 64 / Q;            // fanin records per wave step
+  const int sub = lane & (Q - 1);  // this lane's subset in the record loop
+  const int rec = lane / Q;        // this lane's record in the step
+  const size_t n = static_cast<size_t>(a.n);
+  if (threadIdx.x == 0) s_qn = 0;
+  __syncthreads();
+
+  auto push = [&](int x) {
+    const int p = atomicAdd(&s_qn, 1);
+    if (p < kLdsQueue) s_q[p] = x;
+    else a.queue[atomicAdd(&ctl->qtail, 1u)] = x;
+  };
+  auto flush = [&]() {
+    __syncthreads();
+    const int cnt = min(s_qn, kLdsQueue);
+    if (threadIdx.x == 0) s_base = cnt ? atomicAdd(&ctl->qtail, static_cast<unsigned>(cnt)) : 0u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) a.queue[s_base + i] = s_q[i];
+    __syncthreads();
+    if (threadIdx.x == 0) s_qn = 0;
+  };
+
+  unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
+  int d = 0;
+  float local_max = 0.0f;  // lanes < Q: subset lane's largest finite |distf|
+  while (true) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.lvlp[d] = static_cast<int>(lb);
+    for (unsigned i = lb + static_cast<unsigned>(wave); i < le; i += static_cast<unsigned>(nwaves)) {
+      const int u = a.queue[i];
+      int du = kIntMax, s = -1;
+      if (lane < Q) {
+        if (d == 0) {
+          if (!m.masks || m.masks[lane * n + u]) du = 0;
+        } else {
+          const unsigned long long key = m.best[static_cast<size_t>(u) * Q + lane];
+          if (key != ~0ull) {
+            du = unpack_cand(key);
+            s = unpack_succ(key);
+          }
+        }
+        const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
+        m.dist[lane * n + u] = du;
+        m.succ[lane * n + u] = s;
+        m.distf[lane * n + u] = df;
+        if (du != kIntMax) local_max = fmaxf(local_max, fabsf(df));
+      }
+      if (lane == 0) a.level[u] = d;
+      const int my_du = __shfl(du, sub);
+      const int b = a.iadjp[u], e = a.iadjp[u + 1];
+      for (int j0 = b; j0 < e; j0 += R) {
+        const int j = j0 + rec;
+        int old = 0, x = 0;
+        if (j < e) {
+          const int2 r = a.iedge[j];
+          x = r.x;
+          if (my_du != kIntMax) {
+            const int wi = static_cast<int>(__int_as_float(r.y) * static_cast<float>(kScaleUp));
+            atomicMin(&m.best[static_cast<size_t>(x) * Q + sub], pack_cand(my_du + wi, u));
+          }
+          if (sub == 0) old = atomicSub(&a.deps[x], 1);
+        }
+        if (j < e && sub == 0 && old == 1) push(x);
+      }
+    }
+    flush();
+    unsigned ne = 0;
+    if (!grid_barrier(ctl, nblocks, &ne)) return;
+    if (ne == le) break;
+    lb = le;
+    le = ne;
+    ++d;
+  }
+  if (lane < Q) atomicMax(&m.maxd[lane], __float_as_uint(local_max));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.lvlp[d + 1] = static_cast<int>(le);
+    ctl->num_levels = le > 0 ? d + 1 : 0;
+    ctl->levelized = le;
+  }
+}
+
 // Deviation records of every settled vertex in fanout order, their smallest
 // delta, and the vertex record without its nsv pointer.  Lists of up to
 // kThreadSort records are sorted here (stable insertion sort by delta);
@@ -487,6 +604,33 @@ int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
   void* params[] = {&a};
   CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_kernel, dim3(grid), dim3(kSfxtBlock), params, 0,
                                      stream));
+  return grid;
+}
+
+void launch_sfxt_init_multi(const SfxtArgs& args, const SfxtMulti& m, hipStream_t stream) {
+  hipLaunchKernelGGL(sfxt_init_multi_kernel, dim3(flat_grid(args.n, device_cus())), dim3(256), 0,
+                     stream, args, m);
+  CPG_HIP(hipGetLastError());
+}
+
+int launch_sfxt_levels_multi(const SfxtArgs& args, const SfxtMulti& m, int blocks,
+                             hipStream_t stream) {
+  if (m.q < 1 || m.q > 64 || (m.q & (m.q - 1)) != 0)
+    throw std::runtime_error("subset count must be a power of two up to 64");
+  int dev = 0, coop = 0;
+  CPG_HIP(hipGetDevice(&dev));
+  CPG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) throw std::runtime_error("device does not support cooperative launch");
+  const int cus = device_cus();
+  int per_cu = 0;
+  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_levels_multi_kernel, kSfxtBlock, 0));
+  if (per_cu < 1) throw std::runtime_error("sfxt kernel cannot be resident");
+  const int grid = std::max(1, std::min(blocks, cus * per_cu));
+  SfxtArgs a = args;
+  SfxtMulti mm = m;
+  void* params[] = {&a, &mm};
+  CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_multi_kernel, dim3(grid), dim3(kSfxtBlock), params,
+                                     0, stream));
   return grid;
 }
 
