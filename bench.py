@@ -158,9 +158,14 @@ def served_section(np, cpgamd, nq=48, load=0.8):
         return good
 
     drain([S.submit(gi, k, m) for gi, k, m in queries])  # first run sizes the store
+    st0 = S.stats()
     s0 = time.perf_counter()
     exact = drain([S.submit(gi, k, m) for gi, k, m in queries])
     served_ms = (time.perf_counter() - s0) * 1e3
+    st1 = S.stats()
+    split = {key: round(st1[key] - st0[key], 2) for key in
+             ("sfxt_ms", "levels_ms", "probe_ms", "round_ms", "extract_ms", "compact_ms",
+              "rounds", "sfxt_groups", "compactions")}
     log("served: saturated %.1f ms, exact %d/%d" % (served_ms, exact, nq))
     rate = load * nq / (plain_ms * 1e-3)  # queries/s offered
     arrive = np.cumsum(gaps) / rate * 1e3  # ms after the start
@@ -202,7 +207,7 @@ def served_section(np, cpgamd, nq=48, load=0.8):
             "saturated": {"served_ms": round(served_ms, 2), "plain_ms": round(plain_ms, 2),
                           "served_queries_per_s": round(nq / (served_ms * 1e-3), 2),
                           "plain_queries_per_s": round(nq / (plain_ms * 1e-3), 2),
-                          "speedup": round(plain_ms / served_ms, 3)},
+                          "speedup": round(plain_ms / served_ms, 3), "served_split": split},
             "poisson": {"offered_queries_per_s": round(rate, 2), "load_vs_plain_capacity": load,
                         "served": {"p50_ms": round(float(np.percentile(ls, 50)), 2),
                                    "p99_ms": round(float(np.percentile(ls, 99)), 2),

@@ -51,7 +51,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "submitted", "admitted", "completed", "rounds", "compactions", "restarts",
         "peak_nodes", "node_cap", "peak_index_bytes", "in_flight", "queued", "sfxt_groups")] + [
-        (n, ctypes.c_double) for n in ("sfxt_ms", "probe_ms", "round_ms", "extract_ms", "compact_ms")]
+        (n, ctypes.c_double) for n in ("sfxt_ms", "levels_ms", "probe_ms", "round_ms",
+                                       "extract_ms", "compact_ms")]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -1335,15 +1335,17 @@ struct cpg_server {
     if (h.g.max_out_degree > kSfxtLdsSort) h.devs_tmp.reserve(m);
     Graph::Multi& M = G.mm;
     M.best.reserve(n * Q); M.dist.reserve(n * Q); M.succ.reserve(n * Q); M.distf.reserve(n * Q);
-    M.masks.reserve(n * Q); M.maxd.reserve(Q); M.ctls.reserve(Q);
-    std::vector<uint8_t> hm(n * Q, 0);
+    const size_t ns = h.g.sinks.size();
+    M.masks.reserve(std::max<size_t>(1, ns * Q)); M.maxd.reserve(Q); M.ctls.reserve(Q);
+    // one flag per sink (in sink order) and subset; padding subsets choose none
+    std::vector<uint8_t> hm(ns * Q, 0);
     for (size_t q = 0; q < ids.size(); ++q) {
       const Query& qq = *queries[ids[q]];
-      if (qq.mask.empty()) std::fill(hm.begin() + q * n, hm.begin() + (q + 1) * n, uint8_t(1));
-      else std::copy(qq.mask.begin(), qq.mask.end(), hm.begin() + q * n);
+      for (size_t i = 0; i < ns; ++i)
+        hm[q * ns + i] = qq.mask.empty() ? 1 : (qq.mask[h.g.sinks[i]] ? 1 : 0);
     }
     const double t0 = now_ms();
-    h2d(M.masks.p, hm.data(), n * Q, h.stream);
+    h2d(M.masks.p, hm.data(), ns * Q, h.stream);
     CPG_HIP(hipMemsetAsync(h.sctl.p, 0, sizeof(SfxtCtl), h.stream));
     CPG_HIP(hipMemsetAsync(M.ctls.p, 0, sizeof(SfxtCtl) * Q, h.stream));
     SfxtArgs a{};
@@ -1378,6 +1380,7 @@ struct cpg_server {
     const int lv = sfxt_lift_levels(c.num_levels);
     h.lift.reserve(static_cast<size_t>(lv) * std::max<size_t>(n, 1));
     stats.sfxt_ms += now_ms() - t0;
+    stats.levels_ms += now_ms() - t0;
     stats.sfxt_groups++;
     for (size_t q = 0; q < ids.size(); ++q) {
       Query& qq = *queries[ids[q]];

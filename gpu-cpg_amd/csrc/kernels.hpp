@@ -50,13 +50,14 @@ struct SfxtArgs {
 };
 
 // Q endpoint subsets relaxed in one level loop (served queries): the
-// levels, queue and deps of SfxtArgs are shared; per subset q its sink
-// mask masks[q*n..], distances dist/succ/distf[q*n..] and max |distf|
-// bits maxd[q]; candidates best[v*Q + q].  Q is a power of two <= 64.
+// levels, queue and deps of SfxtArgs are shared; per subset q its chosen
+// sinks masks[q*nsinks + i] (i = position in the sink list), distances
+// dist/succ/distf[q*n..] and max |distf| bits maxd[q]; candidates
+// best[v*Q + q].  Q is a power of two <= 64.
 struct SfxtMulti {
   int q;
   int pad;
-  const uint8_t* masks;       // Q x n bytes, nonzero = chosen sink
+  const uint8_t* masks;       // Q x nsinks bytes, nonzero = chosen sink (null = all)
   unsigned long long* best;   // n x Q
   int* dist;                  // Q x n
   int* succ;                  // Q x n
@@ -159,7 +160,6 @@ struct MqQuery {
   uint32_t shift;                 // histogram window shift
   unsigned long long k;           // paths asked for
   unsigned long long batch_target;
-  unsigned long long expanded;
   int done;                       // nothing left to expand (or the slot is free)
   int rounds;
   // the query's deviation index: the batched designs share the union's,

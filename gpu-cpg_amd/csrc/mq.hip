@@ -162,7 +162,6 @@ __global__ void __launch_bounds__(kMqBlock) mq_make_batch_kernel(MqArgs a) {
     if (take) {
       a.nodes.flag[i] = 1;
       a.batch[slot] = static_cast<uint32_t>(i);
-      atomicAdd(&a.qs[a.nq[i]].expanded, 1ull);
     }
   }
 }
@@ -275,7 +274,16 @@ __global__ void __launch_bounds__(kMqBlock) mq_build_keys_kernel(MqArgs a, const
       key[slot] = (static_cast<uint64_t>(q) << 48) | (static_cast<uint64_t>(k) << 16) |
                   static_cast<uint32_t>(a.nodes.level[i] & 0xffff);
       val[slot] = static_cast<uint32_t>(i);
-      atomicAdd(&per_q[q], 1ull);
+    }
+    // per-query counts: one atomic per distinct query among the wave's
+    // taken lanes (consecutive nodes mostly share a query)
+    uint64_t pend = __ballot(take);
+    while (pend) {
+      const int leader = __ffsll(static_cast<long long>(pend)) - 1;
+      const int lq = __shfl(q, leader);
+      const uint64_t peers = __ballot(take && q == lq);
+      if (__lane_id() == leader) atomicAdd(&per_q[lq], static_cast<unsigned long long>(__popcll(peers)));
+      pend &= ~peers;
     }
   }
 }

@@ -319,8 +319,8 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_multi_kernel(SfxtArgs 
       const int u = a.queue[i];
       int du = kIntMax, s = -1;
       if (lane < Q) {
-        if (d == 0) {
-          if (!m.masks || m.masks[lane * n + u]) du = 0;
+        if (d == 0) {  // level 0 is the sink list in order: i is the sink's position
+          if (!m.masks || m.masks[static_cast<size_t>(lane) * a.nsinks + i]) du = 0;
         } else {
           const unsigned long long key = m.best[static_cast<size_t>(u) * Q + lane];
           if (key != ~0ull) {

@@ -190,7 +190,9 @@ typedef struct {
   int64_t peak_index_bytes;  /* in-flight deviation indexes, device bytes    */
   int64_t in_flight, queued;
   int64_t sfxt_groups;       /* level loops run (each for a group of queries)  */
-  double sfxt_ms, probe_ms, round_ms, extract_ms, compact_ms;  /* host wall time */
+  double sfxt_ms;            /* group level loops + per-query index passes    */
+  double levels_ms;          /* of which the group level loops                */
+  double probe_ms, round_ms, extract_ms, compact_ms;  /* host wall time      */
 } cpg_server_stats;
 /* budget_bytes: device bytes for in-flight indexes (0 = 64 GiB);
  * max_in_flight: query slots, at most 256 (0 = 64); initial_nodes: first
