@@ -165,7 +165,7 @@ def served_section(np, cpgamd, nq=48, load=0.8):
     st1 = S.stats()
     split = {key: round(st1[key] - st0[key], 2) for key in
              ("sfxt_ms", "levels_ms", "probe_ms", "round_ms", "extract_ms", "compact_ms",
-              "rounds", "sfxt_groups", "compactions")}
+              "rounds", "sfxt_groups", "compactions", "root_bounds")}
     log("served: saturated %.1f ms, exact %d/%d" % (served_ms, exact, nq))
     rate = load * nq / (plain_ms * 1e-3)  # queries/s offered
     arrive = np.cumsum(gaps) / rate * 1e3  # ms after the start

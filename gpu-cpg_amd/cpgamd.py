@@ -50,7 +50,7 @@ class Timing(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "submitted", "admitted", "completed", "rounds", "compactions", "restarts",
-        "peak_nodes", "node_cap", "peak_index_bytes", "in_flight", "queued", "sfxt_groups")] + [
+        "peak_nodes", "node_cap", "peak_index_bytes", "in_flight", "queued", "sfxt_groups", "root_bounds")] + [
         (n, ctypes.c_double) for n in ("sfxt_ms", "levels_ms", "probe_ms", "round_ms",
                                        "extract_ms", "compact_ms")]
 

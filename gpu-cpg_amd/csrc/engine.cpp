@@ -1249,7 +1249,34 @@ struct cpg_server {
     const uint32_t klo = nkept > 0 ? float_key(minc) : 0u;
     uint32_t u0 = kKeyInf;
     q.seed_n = 0;
-    if (nkept > 0 && q.k > 0) {
+    // At least k nonzero roots: the k-th smallest root cost bounds the
+    // answer already (roots are paths), and the rounds start from the
+    // roots alone, unexpanded, with no probe walks.
+    std::vector<float> rc;
+    for (int i = 0; i < nsrc; ++i)
+      if (kp[i]) {
+        const float c = static_cast<float>(sd[i]) / static_cast<float>(kScaleUp);
+        if (c != 0.0f) rc.push_back(c);
+      }
+    const bool roots_bound = q.k > 0 && static_cast<int64_t>(rc.size()) >= q.k;
+    if (roots_bound) {
+      std::nth_element(rc.begin(), rc.begin() + (q.k - 1), rc.end());
+      const uint64_t cap = std::max<uint64_t>(uint64_t(1) << 16, static_cast<uint64_t>(nkept));
+      h.ensure_pfxt_buffers(std::max<uint64_t>(cap, h.node_cap));
+      PfxtCtl init{};
+      init.ukey = kKeyInf;
+      init.node_cap = h.node_cap;
+      h2d(h.pctl.p, &init, 1, h.stream);
+      h.keep.reserve(std::max(1, nsrc));
+      h2d(h.keep.p, kp.data(), nsrc, h.stream);
+      launch_roots(h.srcs.p, h.keep.p, nsrc, aq.dist, aq.distf, h.node_store(), h.pctl.p, h.stream);
+      PfxtCtl pc{};
+      d2h(&pc, h.pctl.p, 1, h.stream);
+      CPG_HIP(hipStreamSynchronize(h.stream));
+      u0 = float_key(rc[q.k - 1]);
+      q.seed_n = pc.node_n;
+      stats.root_bounds++;
+    } else if (nkept > 0 && q.k > 0) {
       const uint64_t cap = std::min<uint64_t>(
           std::max<uint64_t>(uint64_t(1) << 22, 16ull * static_cast<uint64_t>(q.k) + nkept + (1u << 20)),
           0x7fffffffull);

@@ -56,6 +56,7 @@ constexpr int kThreadSort = 16;    // deviation lists up to this length: one thr
 constexpr int kWaveSort = 64;      // up to this length: one wave ranks them in registers
 constexpr int kLdsSort = kSfxtLdsSort;  // up to this length: one block sorts in LDS
 constexpr int kFaninUnroll = 8;   // fanin records (and their atomics) in flight per step
+constexpr int kMultiSteps = 4;    // multi-subset loop: record steps in flight per wave
 constexpr int kSortBlock = 256;
 
 __device__ inline unsigned atomic_load_rlx(unsigned* p) {
@@ -337,19 +338,30 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_multi_kernel(SfxtArgs 
       if (lane == 0) a.level[u] = d;
       const int my_du = __shfl(du, sub);
       const int b = a.iadjp[u], e = a.iadjp[u + 1];
-      for (int j0 = b; j0 < e; j0 += R) {
-        const int j = j0 + rec;
-        int old = 0, x = 0;
-        if (j < e) {
-          const int2 r = a.iedge[j];
-          x = r.x;
-          if (my_du != kIntMax) {
-            const int wi = static_cast<int>(__int_as_float(r.y) * static_cast<float>(kScaleUp));
-            atomicMin(&m.best[static_cast<size_t>(x) * Q + sub], pack_cand(my_du + wi, u));
-          }
-          if (sub == 0) old = atomicSub(&a.deps[x], 1);
+      // kMultiSteps record steps per iteration: their loads and atomics are
+      // all in flight before the releases' old values are consumed
+      for (int j0 = b; j0 < e; j0 += kMultiSteps * R) {
+        int2 r[kMultiSteps];
+#pragma unroll
+        for (int t = 0; t < kMultiSteps; ++t) {
+          const int j = j0 + t * R + rec;
+          r[t] = j < e ? a.iedge[j] : make_int2(0, 0);
         }
-        if (j < e && sub == 0 && old == 1) push(x);
+        int old[kMultiSteps];
+#pragma unroll
+        for (int t = 0; t < kMultiSteps; ++t) {
+          old[t] = 0;
+          if (j0 + t * R + rec < e) {
+            if (my_du != kIntMax) {
+              const int wi = static_cast<int>(__int_as_float(r[t].y) * static_cast<float>(kScaleUp));
+              atomicMin(&m.best[static_cast<size_t>(r[t].x) * Q + sub], pack_cand(my_du + wi, u));
+            }
+            if (sub == 0) old[t] = atomicSub(&a.deps[r[t].x], 1);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kMultiSteps; ++t)
+          if (j0 + t * R + rec < e && sub == 0 && old[t] == 1) push(r[t].x);
       }
     }
     flush();

@@ -190,6 +190,7 @@ typedef struct {
   int64_t peak_index_bytes;  /* in-flight deviation indexes, device bytes    */
   int64_t in_flight, queued;
   int64_t sfxt_groups;       /* level loops run (each for a group of queries)  */
+  int64_t root_bounds;       /* queries whose first bound came from their roots */
   double sfxt_ms;            /* group level loops + per-query index passes    */
   double levels_ms;          /* of which the group level loops                */
   double probe_ms, round_ms, extract_ms, compact_ms;  /* host wall time      */
