@@ -356,7 +356,7 @@ struct cpg_handle {
       const int lv = sfxt_lift_levels(c.num_levels);
       lift.reserve(static_cast<size_t>(lv) * n);
       a.lift = lift.p;
-      launch_sfxt_index(a, lv, stream);
+      launch_sfxt_index(a, lv, g.m, stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_sfxt.b, stream));
     }
     d2h(&c, sctl.p, 1, stream);
@@ -1432,7 +1432,7 @@ struct cpg_server {
       std::memcpy(&mad, &maxd[q], 4);
       if (n > 0) {
         const double ti = now_ms();
-        launch_sfxt_index(aq, lv, h.stream);
+        launch_sfxt_index(aq, lv, h.g.m, h.stream);
         SfxtCtl cq{};
         d2h(&cq, aq.ctl, 1, h.stream);
         CPG_HIP(hipStreamSynchronize(h.stream));

@@ -73,7 +73,8 @@ int launch_sfxt_levels_multi(const SfxtArgs& args, const SfxtMulti& m, int block
 void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);
 // Deviation records, their sort, the lift tables and the nsv pointers.
-void launch_sfxt_index(const SfxtArgs& args, int lift_levels, hipStream_t stream);
+// m: the graph's edge count (chooses thread- or wave-per-vertex records).
+void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream);
 constexpr int kSfxtLdsSort = 2048;  // longest deviation list sorted in LDS
 
 // ---------------- PFXT ----------------

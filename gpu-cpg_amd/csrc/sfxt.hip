@@ -450,6 +450,76 @@ __device__ inline uint32_t delta_rank_key(int bits) {
   return x == 0.0f ? kKeyPosZero : float_key(x);
 }
 
+// Deviation records with one wave per vertex, for graphs whose fanouts
+// are long on average: lanes take the fanout edges 64 at a time and a
+// ballot compacts the deviation records in fanout order.  A fanout of at
+// most 64 edges keeps its records in registers, ranks them there (ties in
+// fanout order, as sfxt_sort_mid_kernel) and stores each once, sorted;
+// longer fanouts store their records in fanout order and are queued for
+// sfxt_sort_big_kernel.  vrec and devinfo as in sfxt_devs_kernel.
+__global__ void __launch_bounds__(256) sfxt_devs_wave_kernel(SfxtArgs a) {
+  const int lane = __lane_id();
+  const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwaves = static_cast<int>((gridDim.x * blockDim.x) >> 6);
+  for (int u = wave; u < a.n; u += nwaves) {
+    const int b = a.oadjp[u], e = a.oadjp[u + 1];
+    if (a.level[u] < 0 || a.dist[u] == kIntMax) {  // unsettled or reaches no chosen sink
+      if (lane == 0) {
+        a.vrec[u] = default_vrec();
+        a.devinfo[u] = make_int2(b, 0);
+      }
+      continue;
+    }
+    const int s = a.succ[u];
+    const float df = a.distf[u];
+    int cnt = 0;
+    float mind = __builtin_huge_valf();
+    int4 rec = make_int4(0, 0, 0, 0);
+    bool ok = false;
+    int pos = 0;
+    for (int j0 = b; j0 < e; j0 += 64) {
+      const int j = j0 + lane;
+      ok = false;
+      if (j < e) {
+        const int w = a.oadj[j];
+        if (w != s && a.dist[w] != kIntMax) {
+          const float fw = a.distf[w];
+          const float we = a.owgt[j];
+          const float delta = static_cast<float>(static_cast<double>(fw) + static_cast<double>(we) -
+                                                 static_cast<double>(df));
+          rec = make_int4(w, __float_as_int(we), __float_as_int(fw), __float_as_int(delta));
+          mind = fminf(mind, delta);
+          ok = true;
+        }
+      }
+      const uint64_t bm = __ballot(ok);
+      pos = cnt + static_cast<int>(__popcll(bm & lanemask_lt()));
+      if (e - b > 64 && ok) a.devs[b + pos] = rec;  // long fanout: fanout order, sorted later
+      cnt += static_cast<int>(__popcll(bm));
+    }
+    for (int o = 32; o > 0; o >>= 1) mind = fminf(mind, __shfl_xor(mind, o));
+    if (e - b <= 64) {
+      // rank among the wave's records: smaller key first, ties by position
+      const uint32_t key = ok ? delta_rank_key(rec.w) : 0xffffffffu;
+      const int mypos = ok ? pos : 64;
+      int rank = 0;
+      for (int jj = 0; jj < 64; ++jj) {
+        const uint32_t kj = __shfl(key, jj);
+        const int pj = __shfl(mypos, jj);
+        rank += (kj < key || (kj == key && pj < mypos)) ? 1 : 0;
+      }
+      if (ok) a.devs[b + rank] = rec;
+    } else if (lane == 0) {
+      const unsigned long long sl = atomicAdd(&a.ctl->nbig_l, 1ull);
+      a.big[a.n - 1 - static_cast<int>(sl)] = u;
+    }
+    if (lane == 0) {
+      a.vrec[u] = make_int4(__float_as_int(mind), -1, s, __float_as_int(df));
+      a.devinfo[u] = make_int2(b, cnt);
+    }
+  }
+}
+
 // Stable sort by delta of the lists of kThreadSort+1 .. kWaveSort records,
 // one wave per list: lane i holds record i and its rank is the number of
 // records that order before it (shuffled keys), so the records land in
@@ -646,10 +716,17 @@ int launch_sfxt_levels_multi(const SfxtArgs& args, const SfxtMulti& m, int block
   return grid;
 }
 
-void launch_sfxt_index(const SfxtArgs& args, int lift_levels, hipStream_t stream) {
+void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream) {
   const int cus = device_cus();
   const int g = flat_grid(args.n, cus);
-  hipLaunchKernelGGL(sfxt_devs_kernel, dim3(g), dim3(256), 0, stream, args);
+  if (args.n > 0 && m >= 8 * static_cast<int64_t>(args.n)) {
+    // mean fanout of 8 or more: one wave per vertex
+    const int64_t wb = (static_cast<int64_t>(args.n) + 3) / 4;
+    const int gw = static_cast<int>(std::min<int64_t>(wb, static_cast<int64_t>(cus) * 64));
+    hipLaunchKernelGGL(sfxt_devs_wave_kernel, dim3(gw), dim3(256), 0, stream, args);
+  } else {
+    hipLaunchKernelGGL(sfxt_devs_kernel, dim3(g), dim3(256), 0, stream, args);
+  }
   CPG_HIP(hipGetLastError());
   hipLaunchKernelGGL(sfxt_sort_mid_kernel, dim3(cus * 8), dim3(256), 0, stream, args);
   CPG_HIP(hipGetLastError());
