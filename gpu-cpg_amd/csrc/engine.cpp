@@ -1609,7 +1609,11 @@ struct cpg_server {
         for (int sl = 0; sl < nslots; ++sl)
           if (slot_query[sl] >= 0 && hq[sl].done) fin.push_back(sl);
         if (!fin.empty()) extract(fin);
-        if (!dead_slots.empty() && node_n > st[cur].cap / 2) compact(st[cur].cap);
+        // every round scans the whole store: drop finished queries' nodes
+        // once they may outweigh the live ones (or the store is half full)
+        if (!dead_slots.empty() && node_n > (uint64_t(1) << 22) &&
+            (node_n > st[cur].cap / 2 || static_cast<int>(dead_slots.size()) >= std::max(1, in_flight)))
+          compact(st[cur].cap);
       }
     }
     stats.in_flight = in_flight;

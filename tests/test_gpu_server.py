@@ -106,3 +106,28 @@ def test_server_rejects_bad_input():
         S.get_slacks(qid, 10)  # not finished yet
     run_stream(S)
     assert len(S.get_slacks(qid, 10)) == 10
+
+
+@pytest.mark.parametrize("nq", [9, 16])
+def test_full_admission_group_on_one_graph(nq):
+    """nq queries on one graph are admitted as one group: one level loop
+    relaxes their endpoint subsets together (16 subsets, padded with empty
+    ones for 9), and every result equals the oracle's."""
+    n, u, v, w = cpgamd.gen_preset(4, 38)  # mean fanout 9: the wave deviation pass too
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    rng = np.random.default_rng(100 + nq)
+    S = cpgamd.Server(max_in_flight=64)
+    S.add_graph(n, u, v, w)
+    sub = {}
+    for i in range(nq):
+        k = int(rng.choice([100, 3000, 30000]))
+        m = None if i == 0 else sink_mask(rng, n, u, float(rng.choice([0.01, 0.3, 0.9])))
+        sub[S.submit(0, k, m)] = (k, m)
+    say(f"case server group of {nq} on one graph")
+    run_stream(S)
+    assert S.stats()["sfxt_groups"] == 1
+    for qid, (k, m) in sub.items():
+        d, s, _, _ = og.sfxt(m)
+        want, _ = og.topk(d, s, k)
+        got = S.get_slacks(qid, k)
+        assert len(got) == len(want) and pyoracle.first_bit_mismatch(want, got) == -1, qid
