@@ -1092,6 +1092,15 @@ struct cpg_server {
       DevBuf<unsigned> maxd;
       DevBuf<SfxtCtl> ctls;
     } mm;
+    // the admission group in progress on this graph's stream
+    bool busy = false;
+    std::vector<int64_t> ids;
+    std::vector<SfxtArgs> args;   // per query: its distance columns and index
+    std::vector<uint8_t> hmask;   // host copy of the subsets' sink flags
+    hipEvent_t ev = nullptr;
+    ~Graph() {
+      if (ev) (void)hipEventDestroy(ev);
+    }
   };
   struct Query {
     int64_t id = 0;
@@ -1125,6 +1134,7 @@ struct cpg_server {
   std::deque<int64_t> queue;  // submitted, not yet admitted
   std::unordered_map<int64_t, std::unique_ptr<Query>> queries;
   std::vector<int64_t> slot_query;  // query id per slot (-1 free)
+  std::vector<uint8_t> slot_reserved;  // held by a query of a group in progress
   std::vector<int> free_slots, dead_slots;
   std::vector<int64_t> completed;   // finished since the last poll
   int64_t next_id = 1;
@@ -1170,6 +1180,7 @@ struct cpg_server {
     CPG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     grid = std::max(1, cus) * 8;
     slot_query.assign(nslots, -1);
+    slot_reserved.assign(nslots, 0);
     for (int i = nslots - 1; i >= 0; --i) free_slots.push_back(i);
     hq.assign(nslots, MqQuery{});
     for (auto& m : hq) m.done = 1;
@@ -1192,6 +1203,7 @@ struct cpg_server {
     G->h = std::make_unique<cpg_handle>();
     G->h->set_graph(el);
     G->h->upload_graph();
+    G->h->run_sfxt(false);  // the level count (the same for every endpoint subset)
     const int64_t n = G->h->g.n, m = G->h->g.m;
     G->index_bytes = 24 * n + 16 * m;  // vrec 16, devinfo 8 per vertex; devs 16 per edge
     graphs.push_back(std::move(G));
@@ -1320,8 +1332,9 @@ struct cpg_server {
     q.init = m;
   }
 
-  // Takes a free slot for q and copies its probe's paths (still in the
-  // graph handle's node store) into the shared store.
+  // Puts q in its reserved slot and copies its first-bound paths (still in
+  // the graph handle's node store) into the shared store.  The server
+  // stream must be idle (no round running).
   void admit_one(Query& q) {
     Graph& G = *graphs[q.graph];
     const uint64_t need = CPG_SERVER_NODES_PER_PATH * static_cast<uint64_t>(q.k) + q.seed_n +
@@ -1330,14 +1343,11 @@ struct cpg_server {
       compact(std::max(st[cur].cap, node_n + 2 * need));
       if (node_n + need > st[cur].cap) compact(2 * (node_n + need));
     }
-    const int slot = free_slots.back();
-    free_slots.pop_back();
-    q.slot = slot;
+    const int slot = q.slot;  // reserved when its group started
+    slot_reserved[slot] = 0;
     q.state = 1;
     q.t_admit = now_ms();
     slot_query[slot] = q.id;
-    index_bytes += G.index_bytes;
-    stats.peak_index_bytes = std::max(stats.peak_index_bytes, index_bytes);
     ++in_flight;
     h2d(qs.p + slot, &q.init, 1, stream);
     if (q.seed_n > 0)
@@ -1348,15 +1358,29 @@ struct cpg_server {
     stats.admitted++;
   }
 
-  // Admits a group of queued queries on one graph: one level loop relaxes
-  // all their endpoint subsets (padded to a power of two with empty ones),
-  // then each query gets its deviation index, its first bound and a slot.
-  void admit_group(int gi, const std::vector<int64_t>& ids) {
+  // Starts an admission group of queued queries on one graph: reserves
+  // their slots and budget, and launches on the graph's own stream (no
+  // host wait) one level loop for all their endpoint subsets (padded to a
+  // power of two with empty ones) and each query's deviation index.  The
+  // level count depends on the graph only, so the lift depth is known from
+  // the graph's first SFXT.  finish_group completes the admission once the
+  // stream's event has fired; rounds of the queries in flight run
+  // meanwhile on the server stream.
+  void start_group(int gi, const std::vector<int64_t>& ids) {
     Graph& G = *graphs[gi];
     cpg_handle& h = *G.h;
     const size_t n = static_cast<size_t>(h.g.n), m = static_cast<size_t>(h.g.m);
     int Q = 1;
     while (Q < static_cast<int>(ids.size())) Q <<= 1;
+    for (int64_t id : ids) {
+      Query& q = *queries[id];
+      q.slot = free_slots.back();
+      free_slots.pop_back();
+      slot_reserved[q.slot] = 1;
+      q.state = 3;
+      index_bytes += G.index_bytes;
+    }
+    stats.peak_index_bytes = std::max(stats.peak_index_bytes, index_bytes);
     h.deps.reserve(n); h.queue.reserve(n); h.lvlp.reserve(n + 2); h.level.reserve(n);
     h.big.reserve(n); h.sctl.reserve(1);
     if (h.g.max_out_degree > kSfxtLdsSort) h.devs_tmp.reserve(m);
@@ -1365,19 +1389,19 @@ struct cpg_server {
     const size_t ns = h.g.sinks.size();
     M.masks.reserve(std::max<size_t>(1, ns * Q)); M.maxd.reserve(Q); M.ctls.reserve(Q);
     // one flag per sink (in sink order) and subset; padding subsets choose none
-    std::vector<uint8_t> hm(ns * Q, 0);
+    G.hmask.assign(ns * Q, 0);  // kept until the group finishes (asynchronous copy)
     for (size_t q = 0; q < ids.size(); ++q) {
       const Query& qq = *queries[ids[q]];
       for (size_t i = 0; i < ns; ++i)
-        hm[q * ns + i] = qq.mask.empty() ? 1 : (qq.mask[h.g.sinks[i]] ? 1 : 0);
+        G.hmask[q * ns + i] = qq.mask.empty() ? 1 : (qq.mask[h.g.sinks[i]] ? 1 : 0);
     }
     const double t0 = now_ms();
-    h2d(M.masks.p, hm.data(), ns * Q, h.stream);
+    h2d(M.masks.p, G.hmask.data(), ns * Q, h.stream);
     CPG_HIP(hipMemsetAsync(h.sctl.p, 0, sizeof(SfxtCtl), h.stream));
     CPG_HIP(hipMemsetAsync(M.ctls.p, 0, sizeof(SfxtCtl) * Q, h.stream));
     SfxtArgs a{};
     a.n = h.g.n;
-    a.nsinks = static_cast<int>(h.g.sinks.size());
+    a.nsinks = static_cast<int>(ns);
     a.oadjp = h.oadjp.p; a.oadj = h.oadj.p; a.owgt = h.owgt.p;
     a.iadjp = h.iadjp.p; a.iedge = h.iedge.p; a.sinks = h.sinks.p;
     a.deps = h.deps.p; a.queue = h.queue.p; a.lvlp = h.lvlp.p; a.level = h.level.p;
@@ -1388,27 +1412,16 @@ struct cpg_server {
     mm.q = Q;
     mm.masks = M.masks.p; mm.best = M.best.p; mm.dist = M.dist.p; mm.succ = M.succ.p;
     mm.distf = M.distf.p; mm.maxd = M.maxd.p;
-    SfxtCtl c{};
-    std::vector<unsigned> maxd(Q, 0u);
+    const int lv = sfxt_lift_levels(h.diameter);
+    h.lift.reserve(static_cast<size_t>(lv) * std::max<size_t>(n, 1));
+    G.args.clear();
     if (n > 0) {
       launch_sfxt_init_multi(a, mm, h.stream);
       // one wave per vertex: about one block per 8 vertices of an average level
-      const int64_t per_level = h.last_diameter > 0 ? static_cast<int64_t>(n) / h.last_diameter : 4096;
+      const int64_t per_level = h.diameter > 0 ? static_cast<int64_t>(n) / h.diameter : 4096;
       const int blocks = static_cast<int>(std::min<int64_t>(128, std::max<int64_t>(16, per_level / 8)));
       launch_sfxt_levels_multi(a, mm, blocks, h.stream);
-      d2h(&c, h.sctl.p, 1, h.stream);
-      d2h(maxd.data(), M.maxd.p, Q, h.stream);
-      CPG_HIP(hipStreamSynchronize(h.stream));
-      if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
     }
-    h.diameter = c.num_levels;
-    h.last_diameter = c.num_levels;
-    h.sfxt_valid = false;  // the handle's own SFXT buffers were not written
-    const int lv = sfxt_lift_levels(c.num_levels);
-    h.lift.reserve(static_cast<size_t>(lv) * std::max<size_t>(n, 1));
-    stats.sfxt_ms += now_ms() - t0;
-    stats.levels_ms += now_ms() - t0;
-    stats.sfxt_groups++;
     for (size_t q = 0; q < ids.size(); ++q) {
       Query& qq = *queries[ids[q]];
       std::unique_ptr<IndexSet> ix;
@@ -1428,20 +1441,50 @@ struct cpg_server {
       aq.vrec = ix->vrec.p; aq.devinfo = ix->devinfo.p; aq.devs = ix->devs.p;
       aq.lift = h.lift.p;
       aq.ctl = M.ctls.p + q;
+      if (n > 0) launch_sfxt_index(aq, lv, h.g.m, h.stream);
+      qq.index = std::move(ix);
+      G.args.push_back(aq);
+    }
+    if (!G.ev) CPG_HIP(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
+    CPG_HIP(hipEventRecord(G.ev, h.stream));
+    G.ids = ids;
+    G.busy = true;
+    stats.sfxt_groups++;
+    stats.sfxt_ms += now_ms() - t0;
+  }
+
+  // Completes a started group whose stream work has finished: checks the
+  // level loop and the indexes, then gives every query its first bound and
+  // admits it.  The server stream must be idle.
+  void finish_group(int gi) {
+    Graph& G = *graphs[gi];
+    cpg_handle& h = *G.h;
+    const double t0 = now_ms();
+    const int Q = static_cast<int>(G.args.size());
+    SfxtCtl c{};
+    std::vector<SfxtCtl> cq(std::max(Q, 1));
+    std::vector<unsigned> maxd(std::max(Q, 1), 0u);
+    if (h.g.n > 0) {
+      d2h(&c, h.sctl.p, 1, h.stream);
+      d2h(cq.data(), G.mm.ctls.p, Q, h.stream);
+      d2h(maxd.data(), G.mm.maxd.p, Q, h.stream);
+    }
+    CPG_HIP(hipStreamSynchronize(h.stream));
+    G.busy = false;
+    if (h.g.n > 0) {
+      if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
+      if (c.num_levels != h.diameter) throw std::runtime_error("SFXT levels differ from the graph's");
+      for (int q = 0; q < Q; ++q) {
+        if (cq[q].nsv_err == 2) throw std::runtime_error("SFXT long deviation list without sort buffer");
+        if (cq[q].nsv_err) throw std::runtime_error("SFXT next-smaller descent found no answer");
+      }
+    }
+    stats.sfxt_ms += now_ms() - t0;
+    for (int q = 0; q < Q; ++q) {
+      Query& qq = *queries[G.ids[q]];
       float mad = 0.0f;
       std::memcpy(&mad, &maxd[q], 4);
-      if (n > 0) {
-        const double ti = now_ms();
-        launch_sfxt_index(aq, lv, h.g.m, h.stream);
-        SfxtCtl cq{};
-        d2h(&cq, aq.ctl, 1, h.stream);
-        CPG_HIP(hipStreamSynchronize(h.stream));
-        if (cq.nsv_err == 2) throw std::runtime_error("SFXT long deviation list without sort buffer");
-        if (cq.nsv_err) throw std::runtime_error("SFXT next-smaller descent found no answer");
-        stats.sfxt_ms += now_ms() - ti;
-      }
-      probe_query(qq, h, aq, mad);
-      qq.index = std::move(ix);
+      probe_query(qq, h, G.args[q], mad);
       admit_one(qq);
     }
   }
@@ -1477,31 +1520,69 @@ struct cpg_server {
     stats.compact_ms += now_ms() - t0;
   }
 
-  // FIFO admission between rounds.  The head query's graph forms a group
-  // with the next queued queries on the same graph (at most 16), as long as
-  // slots are free and the in-flight indexes stay within the budget (the
-  // first query is admitted whatever its size).
+  // Admission between rounds (the server stream is idle).  Groups whose
+  // stream work has finished are admitted; then, in FIFO order, each graph
+  // without a group in progress starts one with the first queued query on
+  // it and the next queued queries on the same graph (at most 16), while
+  // slots are free and the in-flight and starting indexes stay within the
+  // budget (a lone query is started whatever its size).  With nothing in
+  // flight, admission waits for the first group in progress.
   void admit() {
+    for (size_t gi = 0; gi < graphs.size(); ++gi) {
+      Graph& G = *graphs[gi];
+      if (!G.busy) continue;
+      const hipError_t e = hipEventQuery(G.ev);
+      if (e == hipSuccess) finish_group(static_cast<int>(gi));
+      else if (e != hipErrorNotReady) CPG_HIP(e);
+    }
     while (!queue.empty()) {
       if (free_slots.empty()) {
         if (dead_slots.empty()) break;
         compact(st[cur].cap);
       }
-      const int gi = queries[queue.front()]->graph;
+      int gi = -1;
+      for (int64_t id : queue)
+        if (!graphs[queries[id]->graph]->busy) {
+          gi = queries[id]->graph;
+          break;
+        }
+      if (gi < 0) break;
       const int64_t gb = graphs[gi]->index_bytes;
       std::vector<int64_t> ids;
       int64_t bytes = index_bytes;
+      const bool idle = in_flight == 0 && !any_busy();
       for (int64_t id : queue) {
         if (static_cast<int>(ids.size()) >= std::min<int>(16, static_cast<int>(free_slots.size()))) break;
         if (queries[id]->graph != gi) continue;
-        if ((in_flight > 0 || !ids.empty()) && bytes + gb > budget) break;  // HBM admission
+        if ((!idle || !ids.empty()) && bytes + gb > budget) break;  // HBM admission
         ids.push_back(id);
         bytes += gb;
       }
       if (ids.empty()) break;
       for (int64_t id : ids) queue.erase(std::find(queue.begin(), queue.end(), id));
-      admit_group(gi, ids);
+      start_group(gi, ids);
     }
+    if (in_flight == 0) {
+      for (size_t gi = 0; gi < graphs.size(); ++gi)
+        if (graphs[gi]->busy) {
+          CPG_HIP(hipEventSynchronize(graphs[gi]->ev));
+          finish_group(static_cast<int>(gi));
+          break;
+        }
+    }
+  }
+
+  bool any_busy() const {
+    for (const auto& G : graphs)
+      if (G->busy) return true;
+    return false;
+  }
+
+  int64_t preparing() const {
+    int64_t k = 0;
+    for (const auto& G : graphs)
+      if (G->busy) k += static_cast<int64_t>(G->ids.size());
+    return k;
   }
 
   // Sorted costs of the finished slots' paths (at or below each bound).
@@ -1573,7 +1654,8 @@ struct cpg_server {
     h2d(qs.p, hq.data(), nslots, stream);
     free_slots.clear();
     dead_slots.clear();
-    for (int i = nslots - 1; i >= 0; --i) free_slots.push_back(i);
+    for (int i = nslots - 1; i >= 0; --i)
+      if (!slot_reserved[i]) free_slots.push_back(i);
     const uint64_t c = std::min<uint64_t>(2 * st[cur].cap, 0x7fffffffull);
     st[cur].reserve(c);
     batch.reserve(c);
@@ -1617,7 +1699,7 @@ struct cpg_server {
       }
     }
     stats.in_flight = in_flight;
-    stats.queued = static_cast<int64_t>(queue.size());
+    stats.queued = static_cast<int64_t>(queue.size()) + preparing();
     stats.node_cap = static_cast<int64_t>(st[cur].cap);
     return stats.in_flight + stats.queued;
   }

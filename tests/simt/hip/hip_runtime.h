@@ -192,6 +192,8 @@ inline T atomicMin(T* p, T v) {
 // ---------------------------------------------------------------- host API
 typedef int hipError_t;
 constexpr hipError_t hipSuccess = 0;
+constexpr hipError_t hipErrorNotReady = 600;
+constexpr unsigned hipEventDisableTiming = 2;
 typedef struct simt_stream* hipStream_t;
 typedef struct simt_event* hipEvent_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
@@ -225,6 +227,10 @@ hipError_t hipEventCreate(hipEvent_t* e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
+// launches run to completion when made, so every recorded event has fired
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
+inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 typedef struct simt_graph* hipGraph_t;
 typedef struct simt_graph* hipGraphExec_t;
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
