@@ -167,7 +167,7 @@ def served_section(np, cpgamd, nq=48, load=0.8):
                 if np.array_equal(a.view(np.uint32), b.view(np.uint32)))
     st1 = S.stats()
     split = {key: round(st1[key] - st0[key], 2) for key in
-             ("sfxt_ms", "levels_ms", "probe_ms", "round_ms", "extract_ms", "compact_ms",
+             ("sfxt_ms", "group_wait_ms", "probe_ms", "round_ms", "extract_ms", "compact_ms",
               "rounds", "sfxt_groups", "compactions", "root_bounds")}
     log("served: saturated %.1f ms, exact %d/%d" % (served_ms, exact, nq))
     rate = load * nq / (plain_ms * 1e-3)  # queries/s offered

@@ -51,7 +51,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "submitted", "admitted", "completed", "rounds", "compactions", "restarts",
         "peak_nodes", "node_cap", "peak_index_bytes", "in_flight", "queued", "sfxt_groups", "root_bounds")] + [
-        (n, ctypes.c_double) for n in ("sfxt_ms", "levels_ms", "probe_ms", "round_ms",
+        (n, ctypes.c_double) for n in ("sfxt_ms", "group_wait_ms", "probe_ms", "round_ms",
                                        "extract_ms", "compact_ms")]
 
     def as_dict(self):

@@ -1565,7 +1565,9 @@ struct cpg_server {
     if (in_flight == 0) {
       for (size_t gi = 0; gi < graphs.size(); ++gi)
         if (graphs[gi]->busy) {
+          const double tw = now_ms();
           CPG_HIP(hipEventSynchronize(graphs[gi]->ev));
+          stats.group_wait_ms += now_ms() - tw;
           finish_group(static_cast<int>(gi));
           break;
         }

@@ -191,8 +191,8 @@ typedef struct {
   int64_t in_flight, queued;
   int64_t sfxt_groups;       /* level loops run (each for a group of queries)  */
   int64_t root_bounds;       /* queries whose first bound came from their roots */
-  double sfxt_ms;            /* group level loops + per-query index passes    */
-  double levels_ms;          /* of which the group level loops                */
+  double sfxt_ms;            /* host time starting and checking groups' SFXTs  */
+  double group_wait_ms;      /* host time waiting for a group, none in flight  */
   double probe_ms, round_ms, extract_ms, compact_ms;  /* host wall time      */
 } cpg_server_stats;
 /* budget_bytes: device bytes for in-flight indexes (0 = 64 GiB);
