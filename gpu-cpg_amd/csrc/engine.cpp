@@ -1096,6 +1096,8 @@ struct cpg_server {
     bool busy = false;
     std::vector<int64_t> ids;
     std::vector<SfxtArgs> args;   // per query: its distance columns and index
+    SfxtArgs base{};              // the group's shared level-loop arguments
+    int lv = 0;                   // lift depth
     std::vector<uint8_t> hmask;   // host copy of the subsets' sink flags
     hipEvent_t ev = nullptr;
     ~Graph() {
@@ -1160,6 +1162,9 @@ struct cpg_server {
   DevBuf<float> out_cost;
   cpg_server_stats stats{};
   bool ready = false;
+  hipStream_t sstream = nullptr;  // level-loop launches of admission groups
+  hipEvent_t ev_levels = nullptr;
+  int parts_cap = 1;              // blocks one partitioned level-loop launch keeps resident
 
   uint64_t first_cap = uint64_t(1) << 24;
 
@@ -1169,12 +1174,16 @@ struct cpg_server {
     if (initial_nodes > 0) first_cap = std::min<uint64_t>(static_cast<uint64_t>(initial_nodes), 0x7fffffffull);
   }
   ~cpg_server() {
+    if (ev_levels) (void)hipEventDestroy(ev_levels);
+    if (sstream) (void)hipStreamDestroy(sstream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
   void init() {
     if (ready) return;
     CPG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    CPG_HIP(hipStreamCreateWithFlags(&sstream, hipStreamNonBlocking));
+    parts_cap = sfxt_parts_capacity();
     int dev = 0, cus = 0;
     CPG_HIP(hipGetDevice(&dev));
     CPG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1358,98 +1367,145 @@ struct cpg_server {
     stats.admitted++;
   }
 
-  // Starts an admission group of queued queries on one graph: reserves
-  // their slots and budget, and launches on the graph's own stream (no
-  // host wait) one level loop for all their endpoint subsets (padded to a
-  // power of two with empty ones) and each query's deviation index.  The
-  // level count depends on the graph only, so the lift depth is known from
-  // the graph's first SFXT.  finish_group completes the admission once the
-  // stream's event has fired; rounds of the queries in flight run
-  // meanwhile on the server stream.
-  void start_group(int gi, const std::vector<int64_t>& ids) {
-    Graph& G = *graphs[gi];
-    cpg_handle& h = *G.h;
-    const size_t n = static_cast<size_t>(h.g.n), m = static_cast<size_t>(h.g.m);
-    int Q = 1;
-    while (Q < static_cast<int>(ids.size())) Q <<= 1;
-    for (int64_t id : ids) {
-      Query& q = *queries[id];
-      q.slot = free_slots.back();
-      free_slots.pop_back();
-      slot_reserved[q.slot] = 1;
-      q.state = 3;
-      index_bytes += G.index_bytes;
+  // Starts admission groups on several idle graphs (at most kMaxParts):
+  // reserves their queries' slots and budget, then on the server's SFXT
+  // stream one cooperative launch runs every group's multi-subset level
+  // loop, each graph on its own partition of blocks with its own barrier
+  // (the launch takes as long as the slowest graph, not the sum).  Each
+  // graph's stream then builds its queries' deviation indexes after the
+  // launch and records the group's event; finish_group completes the
+  // admission once it has fired, while rounds of the queries in flight run
+  // on the server stream meanwhile.  The level count depends on the graph
+  // only, so the lift depth is known from the graph's first SFXT.
+  void start_groups(const std::vector<std::pair<int, std::vector<int64_t>>>& list) {
+    const double t0 = now_ms();
+    SfxtParts parts{};
+    std::vector<int> part_of(list.size(), -1);
+    for (size_t li = 0; li < list.size(); ++li) {
+      const int gi = list[li].first;
+      const std::vector<int64_t>& ids = list[li].second;
+      Graph& G = *graphs[gi];
+      cpg_handle& h = *G.h;
+      const size_t n = static_cast<size_t>(h.g.n), m = static_cast<size_t>(h.g.m);
+      int Q = 1;
+      while (Q < static_cast<int>(ids.size())) Q <<= 1;
+      for (int64_t id : ids) {
+        Query& q = *queries[id];
+        q.slot = free_slots.back();
+        free_slots.pop_back();
+        slot_reserved[q.slot] = 1;
+        q.state = 3;
+        index_bytes += G.index_bytes;
+      }
+      h.deps.reserve(n); h.queue.reserve(n); h.lvlp.reserve(n + 2); h.level.reserve(n);
+      h.big.reserve(n); h.sctl.reserve(1);
+      if (h.g.max_out_degree > kSfxtLdsSort) h.devs_tmp.reserve(m);
+      Graph::Multi& M = G.mm;
+      M.best.reserve(n * Q); M.dist.reserve(n * Q); M.succ.reserve(n * Q); M.distf.reserve(n * Q);
+      const size_t ns = h.g.sinks.size();
+      M.masks.reserve(std::max<size_t>(1, ns * Q)); M.maxd.reserve(Q); M.ctls.reserve(Q);
+      // one flag per sink (in sink order) and subset; padding subsets choose none
+      G.hmask.assign(ns * Q, 0);  // kept until the group finishes (asynchronous copy)
+      for (size_t q = 0; q < ids.size(); ++q) {
+        const Query& qq = *queries[ids[q]];
+        for (size_t i = 0; i < ns; ++i)
+          G.hmask[q * ns + i] = qq.mask.empty() ? 1 : (qq.mask[h.g.sinks[i]] ? 1 : 0);
+      }
+      h2d(M.masks.p, G.hmask.data(), ns * Q, sstream);
+      CPG_HIP(hipMemsetAsync(h.sctl.p, 0, sizeof(SfxtCtl), sstream));
+      CPG_HIP(hipMemsetAsync(M.ctls.p, 0, sizeof(SfxtCtl) * Q, sstream));
+      SfxtArgs a{};
+      a.n = h.g.n;
+      a.nsinks = static_cast<int>(ns);
+      a.oadjp = h.oadjp.p; a.oadj = h.oadj.p; a.owgt = h.owgt.p;
+      a.iadjp = h.iadjp.p; a.iedge = h.iedge.p; a.sinks = h.sinks.p;
+      a.deps = h.deps.p; a.queue = h.queue.p; a.lvlp = h.lvlp.p; a.level = h.level.p;
+      a.big = h.big.p;
+      a.devs_tmp = h.g.max_out_degree > kSfxtLdsSort ? h.devs_tmp.p : nullptr;
+      a.ctl = h.sctl.p;
+      SfxtMulti mm{};
+      mm.q = Q;
+      mm.masks = M.masks.p; mm.best = M.best.p; mm.dist = M.dist.p; mm.succ = M.succ.p;
+      mm.distf = M.distf.p; mm.maxd = M.maxd.p;
+      const int lv = sfxt_lift_levels(h.diameter);
+      h.lift.reserve(static_cast<size_t>(lv) * std::max<size_t>(n, 1));
+      G.args.clear();
+      G.base = a;
+      G.lv = lv;
+      if (n > 0) {
+        launch_sfxt_init_multi(a, mm, sstream);
+        // one wave per vertex: about one block per 8 vertices of an average level
+        const int64_t per_level = h.diameter > 0 ? static_cast<int64_t>(n) / h.diameter : 4096;
+        SfxtPart& P = parts.p[parts.n];
+        P.a = a;
+        P.m = mm;
+        P.nb = static_cast<int>(std::min<int64_t>(128, std::max<int64_t>(16, per_level / 8)));
+        part_of[li] = parts.n++;
+      }
+      G.ids = ids;
     }
     stats.peak_index_bytes = std::max(stats.peak_index_bytes, index_bytes);
-    h.deps.reserve(n); h.queue.reserve(n); h.lvlp.reserve(n + 2); h.level.reserve(n);
-    h.big.reserve(n); h.sctl.reserve(1);
-    if (h.g.max_out_degree > kSfxtLdsSort) h.devs_tmp.reserve(m);
-    Graph::Multi& M = G.mm;
-    M.best.reserve(n * Q); M.dist.reserve(n * Q); M.succ.reserve(n * Q); M.distf.reserve(n * Q);
-    const size_t ns = h.g.sinks.size();
-    M.masks.reserve(std::max<size_t>(1, ns * Q)); M.maxd.reserve(Q); M.ctls.reserve(Q);
-    // one flag per sink (in sink order) and subset; padding subsets choose none
-    G.hmask.assign(ns * Q, 0);  // kept until the group finishes (asynchronous copy)
-    for (size_t q = 0; q < ids.size(); ++q) {
-      const Query& qq = *queries[ids[q]];
-      for (size_t i = 0; i < ns; ++i)
-        G.hmask[q * ns + i] = qq.mask.empty() ? 1 : (qq.mask[h.g.sinks[i]] ? 1 : 0);
-    }
-    const double t0 = now_ms();
-    h2d(M.masks.p, G.hmask.data(), ns * Q, h.stream);
-    CPG_HIP(hipMemsetAsync(h.sctl.p, 0, sizeof(SfxtCtl), h.stream));
-    CPG_HIP(hipMemsetAsync(M.ctls.p, 0, sizeof(SfxtCtl) * Q, h.stream));
-    SfxtArgs a{};
-    a.n = h.g.n;
-    a.nsinks = static_cast<int>(ns);
-    a.oadjp = h.oadjp.p; a.oadj = h.oadj.p; a.owgt = h.owgt.p;
-    a.iadjp = h.iadjp.p; a.iedge = h.iedge.p; a.sinks = h.sinks.p;
-    a.deps = h.deps.p; a.queue = h.queue.p; a.lvlp = h.lvlp.p; a.level = h.level.p;
-    a.big = h.big.p;
-    a.devs_tmp = h.g.max_out_degree > kSfxtLdsSort ? h.devs_tmp.p : nullptr;
-    a.ctl = h.sctl.p;
-    SfxtMulti mm{};
-    mm.q = Q;
-    mm.masks = M.masks.p; mm.best = M.best.p; mm.dist = M.dist.p; mm.succ = M.succ.p;
-    mm.distf = M.distf.p; mm.maxd = M.maxd.p;
-    const int lv = sfxt_lift_levels(h.diameter);
-    h.lift.reserve(static_cast<size_t>(lv) * std::max<size_t>(n, 1));
-    G.args.clear();
-    if (n > 0) {
-      launch_sfxt_init_multi(a, mm, h.stream);
-      // one wave per vertex: about one block per 8 vertices of an average level
-      const int64_t per_level = h.diameter > 0 ? static_cast<int64_t>(n) / h.diameter : 4096;
-      const int blocks = static_cast<int>(std::min<int64_t>(128, std::max<int64_t>(16, per_level / 8)));
-      launch_sfxt_levels_multi(a, mm, blocks, h.stream);
-    }
-    for (size_t q = 0; q < ids.size(); ++q) {
-      Query& qq = *queries[ids[q]];
-      std::unique_ptr<IndexSet> ix;
-      if (!G.pool.empty()) {
-        ix = std::move(G.pool.back());
-        G.pool.pop_back();
-      } else {
-        ix = std::make_unique<IndexSet>();
+    if (parts.n > 0) {
+      // fit every partition into one resident launch (admission forms at
+      // most parts_cap groups, so one block each always fits)
+      const int cap = parts_cap;
+      int total = 0;
+      for (int i = 0; i < parts.n; ++i) total += parts.p[i].nb;
+      if (total > cap) {
+        for (int i = 0; i < parts.n; ++i)
+          parts.p[i].nb = std::max(1, static_cast<int>(static_cast<int64_t>(parts.p[i].nb) * cap / total));
+        for (;;) {
+          int sum = 0, big = 0;
+          for (int i = 0; i < parts.n; ++i) {
+            sum += parts.p[i].nb;
+            if (parts.p[i].nb > parts.p[big].nb) big = i;
+          }
+          if (sum <= cap || parts.p[big].nb <= 1) break;
+          --parts.p[big].nb;
+        }
       }
-      ix->vrec.reserve(std::max<size_t>(n, 1));
-      ix->devinfo.reserve(std::max<size_t>(n, 1));
-      ix->devs.reserve(std::max<size_t>(m, 1));
-      SfxtArgs aq = a;
-      aq.dist = M.dist.p + q * n;
-      aq.succ = M.succ.p + q * n;
-      aq.distf = M.distf.p + q * n;
-      aq.vrec = ix->vrec.p; aq.devinfo = ix->devinfo.p; aq.devs = ix->devs.p;
-      aq.lift = h.lift.p;
-      aq.ctl = M.ctls.p + q;
-      if (n > 0) launch_sfxt_index(aq, lv, h.g.m, h.stream);
-      qq.index = std::move(ix);
-      G.args.push_back(aq);
+      total = 0;
+      for (int i = 0; i < parts.n; ++i) {
+        parts.p[i].b0 = total;
+        total += parts.p[i].nb;
+      }
+      launch_sfxt_levels_parts(parts, total, sstream);
     }
-    if (!G.ev) CPG_HIP(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
-    CPG_HIP(hipEventRecord(G.ev, h.stream));
-    G.ids = ids;
-    G.busy = true;
-    stats.sfxt_groups++;
+    if (!ev_levels) CPG_HIP(hipEventCreateWithFlags(&ev_levels, hipEventDisableTiming));
+    CPG_HIP(hipEventRecord(ev_levels, sstream));
+    for (size_t li = 0; li < list.size(); ++li) {
+      Graph& G = *graphs[list[li].first];
+      cpg_handle& h = *G.h;
+      const size_t n = static_cast<size_t>(h.g.n), m = static_cast<size_t>(h.g.m);
+      CPG_HIP(hipStreamWaitEvent(h.stream, ev_levels, 0));
+      for (size_t q = 0; q < G.ids.size(); ++q) {
+        Query& qq = *queries[G.ids[q]];
+        std::unique_ptr<IndexSet> ix;
+        if (!G.pool.empty()) {
+          ix = std::move(G.pool.back());
+          G.pool.pop_back();
+        } else {
+          ix = std::make_unique<IndexSet>();
+        }
+        ix->vrec.reserve(std::max<size_t>(n, 1));
+        ix->devinfo.reserve(std::max<size_t>(n, 1));
+        ix->devs.reserve(std::max<size_t>(m, 1));
+        SfxtArgs aq = G.base;
+        aq.dist = G.mm.dist.p + q * n;
+        aq.succ = G.mm.succ.p + q * n;
+        aq.distf = G.mm.distf.p + q * n;
+        aq.vrec = ix->vrec.p; aq.devinfo = ix->devinfo.p; aq.devs = ix->devs.p;
+        aq.lift = h.lift.p;
+        aq.ctl = G.mm.ctls.p + q;
+        if (n > 0) launch_sfxt_index(aq, G.lv, h.g.m, h.stream);
+        qq.index = std::move(ix);
+        G.args.push_back(aq);
+      }
+      if (!G.ev) CPG_HIP(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
+      CPG_HIP(hipEventRecord(G.ev, h.stream));
+      G.busy = true;
+      stats.sfxt_groups++;
+    }
     stats.sfxt_ms += now_ms() - t0;
   }
 
@@ -1521,12 +1577,13 @@ struct cpg_server {
   }
 
   // Admission between rounds (the server stream is idle).  Groups whose
-  // stream work has finished are admitted; then, in FIFO order, each graph
-  // without a group in progress starts one with the first queued query on
+  // stream work has finished are admitted.  Then, in FIFO order, each graph
+  // without a group in progress forms one with the first queued query on
   // it and the next queued queries on the same graph (at most 16), while
   // slots are free and the in-flight and starting indexes stay within the
-  // budget (a lone query is started whatever its size).  With nothing in
-  // flight, admission waits for the first group in progress.
+  // budget (a lone query is taken whatever its size); the groups formed
+  // start together (one level-loop launch).  With nothing in flight,
+  // admission waits for the first group in progress.
   void admit() {
     for (size_t gi = 0; gi < graphs.size(); ++gi) {
       Graph& G = *graphs[gi];
@@ -1535,24 +1592,32 @@ struct cpg_server {
       if (e == hipSuccess) finish_group(static_cast<int>(gi));
       else if (e != hipErrorNotReady) CPG_HIP(e);
     }
-    while (!queue.empty()) {
-      if (free_slots.empty()) {
+    std::vector<std::pair<int, std::vector<int64_t>>> batch;
+    std::vector<uint8_t> in_batch(graphs.size(), 0);
+    int held = 0;             // slots promised to the groups formed so far
+    int64_t held_bytes = 0;
+    while (!queue.empty() && static_cast<int>(batch.size()) < std::min(kMaxParts, parts_cap)) {
+      if (static_cast<int>(free_slots.size()) - held <= 0) {
         if (dead_slots.empty()) break;
         compact(st[cur].cap);
+        continue;
       }
       int gi = -1;
-      for (int64_t id : queue)
-        if (!graphs[queries[id]->graph]->busy) {
-          gi = queries[id]->graph;
+      for (int64_t id : queue) {
+        const int g = queries[id]->graph;
+        if (!graphs[g]->busy && !in_batch[g]) {
+          gi = g;
           break;
         }
+      }
       if (gi < 0) break;
       const int64_t gb = graphs[gi]->index_bytes;
+      const bool idle = in_flight == 0 && !any_busy() && batch.empty();
+      const int avail = static_cast<int>(free_slots.size()) - held;
       std::vector<int64_t> ids;
-      int64_t bytes = index_bytes;
-      const bool idle = in_flight == 0 && !any_busy();
+      int64_t bytes = index_bytes + held_bytes;
       for (int64_t id : queue) {
-        if (static_cast<int>(ids.size()) >= std::min<int>(16, static_cast<int>(free_slots.size()))) break;
+        if (static_cast<int>(ids.size()) >= std::min(16, avail)) break;
         if (queries[id]->graph != gi) continue;
         if ((!idle || !ids.empty()) && bytes + gb > budget) break;  // HBM admission
         ids.push_back(id);
@@ -1560,8 +1625,12 @@ struct cpg_server {
       }
       if (ids.empty()) break;
       for (int64_t id : ids) queue.erase(std::find(queue.begin(), queue.end(), id));
-      start_group(gi, ids);
+      held += static_cast<int>(ids.size());
+      held_bytes += gb * static_cast<int64_t>(ids.size());
+      in_batch[gi] = 1;
+      batch.emplace_back(gi, std::move(ids));
     }
+    if (!batch.empty()) start_groups(batch);
     if (in_flight == 0) {
       for (size_t gi = 0; gi < graphs.size(); ++gi)
         if (graphs[gi]->busy) {

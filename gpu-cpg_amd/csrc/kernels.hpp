@@ -68,7 +68,24 @@ struct SfxtMulti {
 // Number of lift levels for a graph of num_levels levels (2^levels > chain length + 1).
 int sfxt_lift_levels(int num_levels);
 void launch_sfxt_init_multi(const SfxtArgs& args, const SfxtMulti& m, hipStream_t stream);
-int launch_sfxt_levels_multi(const SfxtArgs& args, const SfxtMulti& m, int blocks, hipStream_t stream);
+// One graph's multi-subset level loop on the blocks [b0, b0 + nb) of a
+// launch holding several graphs' loops (partitions in ascending b0).
+struct SfxtPart {
+  SfxtArgs a;
+  SfxtMulti m;
+  int b0;
+  int nb;
+};
+constexpr int kMaxParts = 8;  // partitions per launch (passed by value: ~2 KB of arguments)
+struct SfxtParts {
+  SfxtPart p[kMaxParts];
+  int n;
+  int pad;
+};
+// Blocks the partitioned level loop can keep resident at once.
+int sfxt_parts_capacity();
+// parts.p[0..n) cover the blocks [0, total_blocks) in ascending b0.
+void launch_sfxt_levels_parts(const SfxtParts& parts, int total_blocks, hipStream_t stream);
 // Per-vertex init, then the cooperative level loop (returns the grid used).
 void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);

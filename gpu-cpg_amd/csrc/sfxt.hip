@@ -279,15 +279,17 @@ __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMu
 // subset l % Q, offering that subset's candidate to the record's fanin;
 // the subset-0 lane of each record releases it.  The per-level grid
 // barrier is shared by all Q subsets, so Q subsets cost about one loop.
-__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_multi_kernel(SfxtArgs a, SfxtMulti m) {
+// It runs on the blocks [b0, b0 + nblk) of a launch that holds the level
+// loops of several graphs (one partition each, with its own barrier).
+__device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigned bid, unsigned nblk) {
   __shared__ int s_q[kLdsQueue];
   __shared__ int s_qn;
   __shared__ unsigned s_base;
   SfxtCtl* ctl = a.ctl;
-  const unsigned nblocks = gridDim.x;
+  const unsigned nblocks = nblk;
   const int lane = __lane_id();
-  const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int nwaves = static_cast<int>((gridDim.x * blockDim.x) >> 6);
+  const int wave = static_cast<int>((bid * blockDim.x + threadIdx.x) >> 6);
+  const int nwaves = static_cast<int>((nblk * blockDim.x) >> 6);
   const int Q = m.q;
   const int R = 64 / Q;            // fanin records per wave step
   const int sub = lane & (Q - 1);  // this lane's subset in the record loop
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_multi_kernel(SfxtArgs 
   int d = 0;
   float local_max = 0.0f;  // lanes < Q: subset lane's largest finite |distf|
   while (true) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.lvlp[d] = static_cast<int>(lb);
+    if (bid == 0 && threadIdx.x == 0) a.lvlp[d] = static_cast<int>(lb);
     for (unsigned i = lb + static_cast<unsigned>(wave); i < le; i += static_cast<unsigned>(nwaves)) {
       const int u = a.queue[i];
       int du = kIntMax, s = -1;
@@ -373,11 +375,27 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_multi_kernel(SfxtArgs 
     ++d;
   }
   if (lane < Q) atomicMax(&m.maxd[lane], __float_as_uint(local_max));
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     a.lvlp[d + 1] = static_cast<int>(le);
     ctl->num_levels = le > 0 ? d + 1 : 0;
     ctl->levelized = le;
   }
+}
+
+// Level loops of several graphs in one cooperative launch: each block
+// runs the loop of the partition its index falls in, and a partition's
+// barrier counts only its own blocks, so every graph advances at its own
+// pace while all of them are resident together.
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_parts_kernel(SfxtParts parts) {
+  __shared__ int s_part;
+  if (threadIdx.x == 0) {
+    int p = 0;
+    while (p + 1 < parts.n && static_cast<int>(blockIdx.x) >= parts.p[p + 1].b0) ++p;
+    s_part = p;
+  }
+  __syncthreads();
+  const SfxtPart P = parts.p[s_part];
+  levels_multi_part(P.a, P.m, blockIdx.x - static_cast<unsigned>(P.b0), static_cast<unsigned>(P.nb));
 }
 
 // Deviation records of every settled vertex in fanout order, their smallest
@@ -695,25 +713,29 @@ void launch_sfxt_init_multi(const SfxtArgs& args, const SfxtMulti& m, hipStream_
   CPG_HIP(hipGetLastError());
 }
 
-int launch_sfxt_levels_multi(const SfxtArgs& args, const SfxtMulti& m, int blocks,
-                             hipStream_t stream) {
-  if (m.q < 1 || m.q > 64 || (m.q & (m.q - 1)) != 0)
-    throw std::runtime_error("subset count must be a power of two up to 64");
+int sfxt_parts_capacity() {
+  int per_cu = 0;
+  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_levels_parts_kernel, kSfxtBlock, 0));
+  if (per_cu < 1) throw std::runtime_error("sfxt kernel cannot be resident");
+  return device_cus() * per_cu;
+}
+
+void launch_sfxt_levels_parts(const SfxtParts& parts, int total_blocks, hipStream_t stream) {
   int dev = 0, coop = 0;
   CPG_HIP(hipGetDevice(&dev));
   CPG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
   if (!coop) throw std::runtime_error("device does not support cooperative launch");
-  const int cus = device_cus();
-  int per_cu = 0;
-  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_levels_multi_kernel, kSfxtBlock, 0));
-  if (per_cu < 1) throw std::runtime_error("sfxt kernel cannot be resident");
-  const int grid = std::max(1, std::min(blocks, cus * per_cu));
-  SfxtArgs a = args;
-  SfxtMulti mm = m;
-  void* params[] = {&a, &mm};
-  CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_multi_kernel, dim3(grid), dim3(kSfxtBlock), params,
-                                     0, stream));
-  return grid;
+  if (parts.n < 1 || parts.n > kMaxParts || total_blocks > sfxt_parts_capacity())
+    throw std::runtime_error("level-loop partitions do not fit one resident launch");
+  for (int i = 0; i < parts.n; ++i) {
+    const int end = i + 1 < parts.n ? parts.p[i + 1].b0 : total_blocks;
+    if (parts.p[i].nb < 1 || parts.p[i].b0 + parts.p[i].nb != end || (i == 0 && parts.p[0].b0 != 0))
+      throw std::runtime_error("level-loop partitions must tile the launch");
+  }
+  SfxtParts pp = parts;
+  void* params[] = {&pp};
+  CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_parts_kernel, dim3(total_blocks), dim3(kSfxtBlock),
+                                     params, 0, stream));
 }
 
 void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream) {

@@ -231,6 +231,7 @@ hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
 inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 typedef struct simt_graph* hipGraph_t;
 typedef struct simt_graph* hipGraphExec_t;
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
