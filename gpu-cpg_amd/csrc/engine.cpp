@@ -1104,22 +1104,6 @@ struct cpg_server {
       if (ev) (void)hipEventDestroy(ev);
     }
   };
-  // A copy of a first-bound probe's paths, taken from the graph handle's
-  // node store so that the handle can probe the next query before this one
-  // is seeded into the shared store.
-  struct Stage {
-    DevBuf<float> cost;
-    DevBuf<int> to, from, parent, level;
-    DevBuf<uint32_t> key;
-    DevBuf<uint8_t> flag;
-    void reserve(uint64_t c) {
-      cost.reserve(c); to.reserve(c); from.reserve(c); parent.reserve(c); level.reserve(c);
-      key.reserve(c); flag.reserve(c);
-    }
-    NodeStore view(uint64_t c) {
-      return NodeStore{cost.p, to.p, from.p, parent.p, level.p, key.p, flag.p, c};
-    }
-  };
   struct Query {
     int64_t id = 0;
     int graph = 0;
@@ -1127,11 +1111,10 @@ struct cpg_server {
     std::vector<uint8_t> mask;  // n bytes, or empty for every sink
     double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
     int slot = -1;
-    int state = 0;              // 0 queued, 1 in flight, 2 done, 3 group in progress, 4 probed
+    int state = 0;              // 0 queued, 1 in flight, 2 done
     std::unique_ptr<IndexSet> index;
     std::vector<int> roots;     // kept roots (vertex ids, ascending)
-    uint64_t seed_n = 0;        // paths of the first-bound probe (staged)
-    std::unique_ptr<Stage> stage;
+    uint64_t seed_n = 0;        // paths of the first-bound probe left in the graph handle's store
     MqQuery init{};             // slot state at admission
     std::vector<float> result;
   };
@@ -1182,8 +1165,6 @@ struct cpg_server {
   hipStream_t sstream = nullptr;  // level-loop launches of admission groups
   hipEvent_t ev_levels = nullptr;
   int parts_cap = 1;              // blocks one partitioned level-loop launch keeps resident
-  std::vector<int64_t> probed;    // queries with a first bound, waiting for admission
-  std::vector<std::unique_ptr<Stage>> stage_pool;
 
   uint64_t first_cap = uint64_t(1) << 24;
 
@@ -1358,26 +1339,6 @@ struct cpg_server {
     m.step_limit = h.step_limit();
     m.all = u0 == kKeyInf ? 1 : 0;
     q.init = m;
-    // stage the probe's paths: the handle probes the next query next (the
-    // copies run in order on the handle's stream before that)
-    if (q.seed_n > 0) {
-      if (!stage_pool.empty()) {
-        q.stage = std::move(stage_pool.back());
-        stage_pool.pop_back();
-      } else {
-        q.stage = std::make_unique<Stage>();
-      }
-      q.stage->reserve(q.seed_n);
-      const NodeStore src = h.node_store();
-      const size_t c = q.seed_n;
-      CPG_HIP(hipMemcpyAsync(q.stage->cost.p, src.cost, 4 * c, hipMemcpyDeviceToDevice, h.stream));
-      CPG_HIP(hipMemcpyAsync(q.stage->to.p, src.to, 4 * c, hipMemcpyDeviceToDevice, h.stream));
-      CPG_HIP(hipMemcpyAsync(q.stage->from.p, src.from, 4 * c, hipMemcpyDeviceToDevice, h.stream));
-      CPG_HIP(hipMemcpyAsync(q.stage->parent.p, src.parent, 4 * c, hipMemcpyDeviceToDevice, h.stream));
-      CPG_HIP(hipMemcpyAsync(q.stage->level.p, src.level, 4 * c, hipMemcpyDeviceToDevice, h.stream));
-      CPG_HIP(hipMemcpyAsync(q.stage->key.p, src.key, 4 * c, hipMemcpyDeviceToDevice, h.stream));
-      CPG_HIP(hipMemcpyAsync(q.stage->flag.p, src.flag, c, hipMemcpyDeviceToDevice, h.stream));
-    }
   }
 
   // Puts q in its reserved slot and copies its first-bound paths (still in
@@ -1399,11 +1360,10 @@ struct cpg_server {
     ++in_flight;
     h2d(qs.p + slot, &q.init, 1, stream);
     if (q.seed_n > 0)
-      launch_mq_seed(q.stage->view(q.seed_n), q.seed_n, st[cur].view(), st[cur].nq.p, node_n, slot,
+      launch_mq_seed(G.h->node_store(), q.seed_n, st[cur].view(), st[cur].nq.p, node_n, slot,
                      mctl.p, grid, stream);
     node_n += q.seed_n;
     CPG_HIP(hipStreamSynchronize(stream));
-    if (q.stage) stage_pool.push_back(std::move(q.stage));
     stats.admitted++;
   }
 
@@ -1550,9 +1510,8 @@ struct cpg_server {
   }
 
   // Completes a started group whose stream work has finished: checks the
-  // level loop and the indexes, then gives every query its first bound
-  // (probe walks on the graph's stream, which may overlap a round) and
-  // queues it for admission.
+  // level loop and the indexes, then gives every query its first bound and
+  // admits it.  The server stream must be idle.
   void finish_group(int gi) {
     Graph& G = *graphs[gi];
     cpg_handle& h = *G.h;
@@ -1582,10 +1541,8 @@ struct cpg_server {
       float mad = 0.0f;
       std::memcpy(&mad, &maxd[q], 4);
       probe_query(qq, h, G.args[q], mad);
-      qq.state = 4;
-      probed.push_back(qq.id);
+      admit_one(qq);
     }
-    CPG_HIP(hipStreamSynchronize(h.stream));  // the last stage copies
   }
 
   // Copies the live nodes (slots not done) into the other buffer set, of
@@ -1619,14 +1576,22 @@ struct cpg_server {
     stats.compact_ms += now_ms() - t0;
   }
 
-  // Starting admission groups (the server stream is idle).  In FIFO
-  // order, each graph
+  // Admission between rounds (the server stream is idle).  Groups whose
+  // stream work has finished are admitted.  Then, in FIFO order, each graph
   // without a group in progress forms one with the first queued query on
   // it and the next queued queries on the same graph (at most 16), while
   // slots are free and the in-flight and starting indexes stay within the
   // budget (a lone query is taken whatever its size); the groups formed
-  // start together (one level-loop launch).
-  void admit_start() {
+  // start together (one level-loop launch).  With nothing in flight,
+  // admission waits for the first group in progress.
+  void admit() {
+    for (size_t gi = 0; gi < graphs.size(); ++gi) {
+      Graph& G = *graphs[gi];
+      if (!G.busy) continue;
+      const hipError_t e = hipEventQuery(G.ev);
+      if (e == hipSuccess) finish_group(static_cast<int>(gi));
+      else if (e != hipErrorNotReady) CPG_HIP(e);
+    }
     std::vector<std::pair<int, std::vector<int64_t>>> batch;
     std::vector<uint8_t> in_batch(graphs.size(), 0);
     int held = 0;             // slots promised to the groups formed so far
@@ -1666,39 +1631,16 @@ struct cpg_server {
       batch.emplace_back(gi, std::move(ids));
     }
     if (!batch.empty()) start_groups(batch);
-  }
-
-  // Probes the queries of every group whose stream work has finished
-  // (overlapping the round in flight, if any).  With wait and none ready,
-  // waits for the first group in progress.
-  void probe_ready(bool wait) {
-    bool any = false;
-    for (size_t gi = 0; gi < graphs.size(); ++gi) {
-      Graph& G = *graphs[gi];
-      if (!G.busy) continue;
-      const hipError_t e = hipEventQuery(G.ev);
-      if (e == hipSuccess) {
-        finish_group(static_cast<int>(gi));
-        any = true;
-      } else if (e != hipErrorNotReady) {
-        CPG_HIP(e);
-      }
+    if (in_flight == 0) {
+      for (size_t gi = 0; gi < graphs.size(); ++gi)
+        if (graphs[gi]->busy) {
+          const double tw = now_ms();
+          CPG_HIP(hipEventSynchronize(graphs[gi]->ev));
+          stats.group_wait_ms += now_ms() - tw;
+          finish_group(static_cast<int>(gi));
+          break;
+        }
     }
-    if (any || !wait) return;
-    for (size_t gi = 0; gi < graphs.size(); ++gi)
-      if (graphs[gi]->busy) {
-        const double tw = now_ms();
-        CPG_HIP(hipEventSynchronize(graphs[gi]->ev));
-        stats.group_wait_ms += now_ms() - tw;
-        finish_group(static_cast<int>(gi));
-        return;
-      }
-  }
-
-  // Admits the probed queries (the server stream must be idle).
-  void admit_probed() {
-    for (int64_t id : probed) admit_one(*queries[id]);
-    probed.clear();
   }
 
   bool any_busy() const {
@@ -1797,23 +1739,16 @@ struct cpg_server {
 
   // Admission, one round for every in-flight query, extraction of the
   // finished ones.  Returns the queries still queued or in flight.
-  // Starts admission groups, runs one round of the queries in flight while
-  // the finished groups' queries are probed, then extracts the finished
-  // queries and admits the probed ones.  Returns the queries still queued,
-  // starting, probed or in flight.
   int64_t step() {
     init();
-    admit_start();
-    const bool round = in_flight > 0;
-    MqCtl mc{};
-    if (round) {
-      launch_mq_round(args(), false, grid, stream);
+    admit();
+    if (in_flight > 0) {
+      const double t0 = now_ms();
+      MqArgs a = args();
+      launch_mq_round(a, false, grid, stream);
+      MqCtl mc{};
       d2h(&mc, mctl.p, 1, stream);
       d2h(hq.data(), qs.p, nslots, stream);
-    }
-    probe_ready(!round && probed.empty());
-    if (round) {
-      const double t0 = now_ms();
       CPG_HIP(hipStreamSynchronize(stream));
       stats.rounds++;
       stats.round_ms += now_ms() - t0;
@@ -1834,9 +1769,8 @@ struct cpg_server {
           compact(st[cur].cap);
       }
     }
-    admit_probed();
     stats.in_flight = in_flight;
-    stats.queued = static_cast<int64_t>(queue.size()) + preparing() + static_cast<int64_t>(probed.size());
+    stats.queued = static_cast<int64_t>(queue.size()) + preparing();
     stats.node_cap = static_cast<int64_t>(st[cur].cap);
     return stats.in_flight + stats.queued;
   }
