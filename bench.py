@@ -23,6 +23,10 @@ Prints one JSON line.  Besides the contract fields it carries
                   SFXT, multi-query rounds replayed from a captured HIP
                   graph) against a loop of plain single queries on resident
                   graphs; queries/s, speedup, exactness
+  bvss            BVSS deviation discovery on the densified graph (30M
+                  edges), 16 dense frontiers: wave64 bitmap path vs int8 MFMA
+                  tile path, kernel ms (HIP events), tiles, achieved int8
+                  TOPS against the 5.0 POPS dense peak, both paths equal
   served          8 resident graphs (200K-2.6M vertices, up to 70M edges),
                   a seeded stream of queries each with an endpoint subset
                   (1-100% of sinks) and K log-uniform 1K-1M: saturated
@@ -110,6 +114,42 @@ def batched_section(np, cpgamd, reps=3):
             "plain_queries_per_s": round(256 / (plain_ms * 1e-3), 1),
             "speedup": round(plain_ms / best, 3), "exact": "%d/256" % exact,
             "batch_latency_ms": round(best, 3)}
+
+
+def bvss_section(np, cpgamd, nfront=16, span=4096, reps=3):
+    """BVSS A_dev discovery on the densified graph: nfront frontiers, each
+    span consecutive source vertices (dense 64-source intervals), by the
+    wave64 bitmap path and by the int8 MFMA tile path."""
+    t0 = time.time()
+    n, u, v, w = cpgamd.gen_preset(3)
+    g = cpgamd.CpGen()
+    g.load_edges(n, u, v, w)
+    g.run_sfxt()
+    t1 = time.time()
+    g.bvss_build()
+    log("bvss: graph %.1fs, index build %.1fs" % (t1 - t0, time.time() - t1))
+    info = g.bvss_info()
+    rng = np.random.default_rng(9)
+    starts = rng.integers(0, n - span, size=nfront) // 64 * 64
+    frontiers = [np.arange(s0, s0 + span, dtype=np.int32) for s0 in starts]
+    out = {"workload": "densified DAG 1.5M vertices / 30M edges; %d frontiers of %d consecutive sources"
+                       % (nfront, span), "entries": info["entries"], "set_bits": info["set_bits"]}
+    res = {}
+    for mode, name in ((1, "bitmap"), (2, "mfma")):
+        ms = []
+        for _ in range(reps):
+            res[name] = g.bvss_discover(frontiers, mode=mode)
+            t = g.timings()
+            ms.append(t["bvss_ms"])
+        out[name] = {"ms": round(min(ms), 4), "triples": int(len(res[name][0]))}
+        if mode == 2:
+            tiles = int(t["bvss_tiles"])
+            ops = 2.0 * 16 * 16 * 64 * tiles
+            tops = ops / (min(ms) * 1e-3) / 1e12
+            out[name].update({"tiles": tiles, "int8_ops": ops, "achieved_TOPS": round(tops, 3),
+                              "peak_TOPS": 5000.0, "frac": round(tops / 5000.0, 6)})
+    out["exact"] = all(np.array_equal(a, b) for a, b in zip(res["bitmap"], res["mfma"]))
+    return out
 
 
 def served_section(np, cpgamd, nq=48, load=0.8):
@@ -231,6 +271,7 @@ def main():
     ap.add_argument("--cpu-trials", type=int, default=2)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-bvss", action="store_true")
     ap.add_argument("--served-queries", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -367,6 +408,8 @@ def main():
         out["batched"] = batched_section(np, cpgamd)
     if not args.no_served:
         out["served"] = served_section(np, cpgamd, args.served_queries)
+    if not args.no_bvss:
+        out["bvss"] = bvss_section(np, cpgamd)
     print(json.dumps(out), flush=True)
 
 
