@@ -10,14 +10,13 @@ namespace cpg {
 // ---------------- SFXT ----------------
 struct SfxtCtl {
   unsigned qtail;               // level-order queue tail
-  unsigned bar_count;           // grid barrier arrivals
-  unsigned bar_gen;             // grid barrier generation
   int err;                      // barrier spin limit exceeded
   int num_levels;               // graph diameter (levels peeled)
   unsigned levelized;           // vertices peeled
   unsigned max_abs_distf_bits;  // max |distf| as float bits
   int nsv_err;                  // 1: nsv descent found no answer, 2: long list without devs_tmp
-  unsigned bar_tail;            // qtail when the last grid barrier completed
+  unsigned long long bar_arrive;  // grid barrier: arrivals << 40 | vertices the arrived blocks pushed
+  unsigned long long bar_word;    // grid barrier: generation << 32 | queue tail when it completed
   unsigned long long nbig;      // deviation lists one wave sorts (front of big[])
   unsigned long long nbig_l;    // deviation lists one block sorts (back of big[])
 };

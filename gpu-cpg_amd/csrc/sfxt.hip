@@ -59,39 +59,51 @@ constexpr int kFaninUnroll = 8;   // fanin records (and their atomics) in flight
 constexpr int kMultiSteps = 4;    // multi-subset loop: record steps in flight per wave
 constexpr int kSortBlock = 256;
 
-__device__ inline unsigned atomic_load_rlx(unsigned* p) {
+__device__ inline unsigned long long atomic_load_rlx64(unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Generation barrier over all co-resident blocks of a cooperative launch.
-// One release fence before arriving and one acquire fence after leaving;
-// the wait polls with relaxed loads so it does not invalidate caches on
-// every probe.  A spin that exceeds kSpinLimit raises ctl->err and every
-// later barrier returns false at once, so the kernel always drains.
-// Returns false on error; *qtail receives the queue tail as it stood when
-// the barrier completed: the last block to arrive (every block's pushes
-// are done) stores it in bar_tail before releasing the others.  Reading
-// qtail itself after leaving would race with blocks already pushing the
-// next level, and blocks could then disagree on the level's end.
-__device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned* qtail) {
+// Generation barrier over the blocks of one level loop, which also hands
+// every block the queue tail of the next level.  One release fence before
+// arriving and one acquire fence after leaving; the wait polls with
+// relaxed loads so it does not invalidate caches on every probe.
+//
+// Each block arrives with the number of vertices it pushed this level
+// (its own queue writes are done by then): the arrival word counts blocks
+// in its top 24 bits and sums their pushes in the low 40.  The last block
+// to arrive therefore knows the level's tail, le + the sum, without
+// reading qtail again, and releases the others with one 64-bit store of
+// the next generation and that tail, which the waiters take from the word
+// they poll.  (Reading qtail itself after leaving would race with blocks
+// already pushing the next level.)  A spin that exceeds kSpinLimit raises
+// ctl->err and every later barrier returns false at once, so the kernel
+// always drains.  Returns false on error.
+__device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned pushed, unsigned le,
+                             unsigned* next_tail) {
   __syncthreads();
   __shared__ int s_ok;
   __shared__ unsigned s_tail;
   if (threadIdx.x == 0) {
     int ok = 1;
-    const unsigned g = atomic_load_rlx(&ctl->bar_gen);
+    unsigned tail = 0;
+    const unsigned g = static_cast<unsigned>(atomic_load_rlx64(&ctl->bar_word) >> 32);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nblocks - 1) {
+    const unsigned long long old = __hip_atomic_fetch_add(
+        &ctl->bar_arrive, (1ull << 40) | pushed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (static_cast<unsigned>(old >> 40) == nblocks - 1) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      __hip_atomic_store(&ctl->bar_tail, atomic_load_rlx(&ctl->qtail), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      tail = le + static_cast<unsigned>(old & ((1ull << 40) - 1)) + pushed;
+      __hip_atomic_store(&ctl->bar_arrive, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->bar_word, (static_cast<unsigned long long>(g + 1u) << 32) | tail,
+                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       unsigned spins = 0;
-      while (atomic_load_rlx(&ctl->bar_gen) == g) {
+      for (;;) {
+        const unsigned long long w = atomic_load_rlx64(&ctl->bar_word);
+        if (static_cast<unsigned>(w >> 32) != g) {
+          tail = static_cast<unsigned>(w);
+          break;
+        }
         if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
         if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); ok = 0; break; }
         __builtin_amdgcn_s_sleep(1);
@@ -99,11 +111,11 @@ __device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned* qtail) {
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
-    s_tail = atomic_load_rlx(&ctl->bar_tail);
+    s_tail = tail;
     s_ok = ok;
   }
   __syncthreads();
-  *qtail = s_tail;
+  *next_tail = s_tail;
   return s_ok != 0;
 }
 
@@ -177,14 +189,18 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
     if (p < kLdsQueue) s_q[p] = x;
     else a.queue[atomicAdd(&ctl->qtail, 1u)] = x;
   };
-  auto flush = [&]() {
+  // Writes the block's LDS queue out; returns every vertex the block
+  // pushed this level (those that spilled past the LDS queue included).
+  auto flush = [&]() -> unsigned {
     __syncthreads();
-    const int cnt = min(s_qn, kLdsQueue);
+    const int total = s_qn;
+    const int cnt = min(total, kLdsQueue);
     if (threadIdx.x == 0) s_base = cnt ? atomicAdd(&ctl->qtail, static_cast<unsigned>(cnt)) : 0u;
     __syncthreads();
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) a.queue[s_base + i] = s_q[i];
     __syncthreads();
     if (threadIdx.x == 0) s_qn = 0;
+    return static_cast<unsigned>(total);
   };
 
   unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
@@ -235,9 +251,9 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
           if (j + q < e && old[q] == 1) push(r[q].x);
       }
     }
-    flush();
+    const unsigned pushed = flush();
     unsigned ne = 0;
-    if (!grid_barrier(ctl, nblocks, &ne)) return;
+    if (!grid_barrier(ctl, nblocks, pushed, le, &ne)) return;
     if (ne == le) break;
     lb = le;
     le = ne;
@@ -303,14 +319,18 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
     if (p < kLdsQueue) s_q[p] = x;
     else a.queue[atomicAdd(&ctl->qtail, 1u)] = x;
   };
-  auto flush = [&]() {
+  // Writes the block's LDS queue out; returns every vertex the block
+  // pushed this level (those that spilled past the LDS queue included).
+  auto flush = [&]() -> unsigned {
     __syncthreads();
-    const int cnt = min(s_qn, kLdsQueue);
+    const int total = s_qn;
+    const int cnt = min(total, kLdsQueue);
     if (threadIdx.x == 0) s_base = cnt ? atomicAdd(&ctl->qtail, static_cast<unsigned>(cnt)) : 0u;
     __syncthreads();
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) a.queue[s_base + i] = s_q[i];
     __syncthreads();
     if (threadIdx.x == 0) s_qn = 0;
+    return static_cast<unsigned>(total);
   };
 
   unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
@@ -366,9 +386,9 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
           if (j0 + t * R + rec < e && sub == 0 && old[t] == 1) push(r[t].x);
       }
     }
-    flush();
+    const unsigned pushed = flush();
     unsigned ne = 0;
-    if (!grid_barrier(ctl, nblocks, &ne)) return;
+    if (!grid_barrier(ctl, nblocks, pushed, le, &ne)) return;
     if (ne == le) break;
     lb = le;
     le = ne;
