@@ -14,7 +14,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <string>
 #include <vector>
@@ -1164,6 +1167,7 @@ struct cpg_server {
   bool ready = false;
   hipStream_t sstream = nullptr;  // level-loop launches of admission groups
   hipEvent_t ev_levels = nullptr;
+  std::mutex stats_mu;  // statistics the probes of several graphs update
   int parts_cap = 1;              // blocks one partitioned level-loop launch keeps resident
 
   uint64_t first_cap = uint64_t(1) << 24;
@@ -1296,6 +1300,7 @@ struct cpg_server {
       CPG_HIP(hipStreamSynchronize(h.stream));
       u0 = float_key(rc[q.k - 1]);
       q.seed_n = pc.node_n;
+      std::lock_guard<std::mutex> lk(stats_mu);
       stats.root_bounds++;
     } else if (nkept > 0 && q.k > 0) {
       const uint64_t cap = std::min<uint64_t>(
@@ -1318,7 +1323,10 @@ struct cpg_server {
       wa.step_limit = h.step_limit();
       const double tp = now_ms();
       u0 = h.probe_bound(wa, q.k, kp, sd, minc, klo, false);
-      stats.probe_ms += now_ms() - tp;
+      {
+        std::lock_guard<std::mutex> lk(stats_mu);
+        stats.probe_ms += now_ms() - tp;
+      }
       PfxtCtl pc{};
       d2h(&pc, h.pctl.p, 1, h.stream);
       CPG_HIP(hipStreamSynchronize(h.stream));
@@ -1373,7 +1381,7 @@ struct cpg_server {
   // loop, each graph on its own partition of blocks with its own barrier
   // (the launch takes as long as the slowest graph, not the sum).  Each
   // graph's stream then builds its queries' deviation indexes after the
-  // launch and records the group's event; finish_group completes the
+  // launch and records the group's event; finish_groups completes the
   // admission once it has fired, while rounds of the queries in flight run
   // on the server stream meanwhile.  The level count depends on the graph
   // only, so the lift depth is known from the graph's first SFXT.
@@ -1509,10 +1517,61 @@ struct cpg_server {
     stats.sfxt_ms += now_ms() - t0;
   }
 
-  // Completes a started group whose stream work has finished: checks the
-  // level loop and the indexes, then gives every query its first bound and
-  // admits it.  The server stream must be idle.
-  void finish_group(int gi) {
+  // Completes started groups whose stream work has finished: checks each
+  // group's level loop and indexes, then gives every query its first bound
+  // and admits it.  The server stream must be idle.  A graph's queries are
+  // probed one after another (a probe leaves its paths in the graph
+  // handle's node store until admission has copied them), but the graphs'
+  // probes run on host threads of their own, each on its graph's stream;
+  // admissions, which use the shared store and the server stream, take a
+  // lock one at a time.
+  void finish_groups(const std::vector<int>& gis) {
+    if (gis.empty()) return;
+    std::vector<std::vector<unsigned>> maxd(gis.size());
+    for (size_t i = 0; i < gis.size(); ++i) maxd[i] = check_group(gis[i]);
+    auto run = [&](size_t i, std::mutex* mu) {
+      Graph& G = *graphs[gis[i]];
+      for (size_t q = 0; q < G.ids.size(); ++q) {
+        Query& qq = *queries[G.ids[q]];
+        float mad = 0.0f;
+        std::memcpy(&mad, &maxd[i][q], 4);
+        probe_query(qq, *G.h, G.args[q], mad);
+        if (mu) {
+          std::lock_guard<std::mutex> lk(*mu);
+          admit_one(qq);
+        } else {
+          admit_one(qq);
+        }
+      }
+    };
+    if (gis.size() == 1) {
+      run(0, nullptr);
+      return;
+    }
+    int dev = 0;
+    CPG_HIP(hipGetDevice(&dev));
+    std::mutex admit_mu;
+    std::vector<std::exception_ptr> errs(gis.size());
+    std::vector<std::thread> th;
+    th.reserve(gis.size());
+    for (size_t i = 0; i < gis.size(); ++i)
+      th.emplace_back([&, i]() {
+        try {
+          CPG_HIP(hipSetDevice(dev));
+          run(i, &admit_mu);
+        } catch (...) {
+          errs[i] = std::current_exception();
+        }
+      });
+    for (auto& t : th) t.join();
+    for (auto& e : errs)
+      if (e) std::rethrow_exception(e);
+  }
+
+  // Checks a started group whose stream work has finished (its level loop
+  // and its queries' indexes) and returns each query's largest finite
+  // |distf| as float bits.
+  std::vector<unsigned> check_group(int gi) {
     Graph& G = *graphs[gi];
     cpg_handle& h = *G.h;
     const double t0 = now_ms();
@@ -1536,13 +1595,7 @@ struct cpg_server {
       }
     }
     stats.sfxt_ms += now_ms() - t0;
-    for (int q = 0; q < Q; ++q) {
-      Query& qq = *queries[G.ids[q]];
-      float mad = 0.0f;
-      std::memcpy(&mad, &maxd[q], 4);
-      probe_query(qq, h, G.args[q], mad);
-      admit_one(qq);
-    }
+    return maxd;
   }
 
   // Copies the live nodes (slots not done) into the other buffer set, of
@@ -1585,13 +1638,7 @@ struct cpg_server {
   // start together (one level-loop launch).  With nothing in flight,
   // admission waits for the first group in progress.
   void admit() {
-    for (size_t gi = 0; gi < graphs.size(); ++gi) {
-      Graph& G = *graphs[gi];
-      if (!G.busy) continue;
-      const hipError_t e = hipEventQuery(G.ev);
-      if (e == hipSuccess) finish_group(static_cast<int>(gi));
-      else if (e != hipErrorNotReady) CPG_HIP(e);
-    }
+    finish_groups(ready_groups(-1));
     std::vector<std::pair<int, std::vector<int64_t>>> batch;
     std::vector<uint8_t> in_batch(graphs.size(), 0);
     int held = 0;             // slots promised to the groups formed so far
@@ -1637,10 +1684,25 @@ struct cpg_server {
           const double tw = now_ms();
           CPG_HIP(hipEventSynchronize(graphs[gi]->ev));
           stats.group_wait_ms += now_ms() - tw;
-          finish_group(static_cast<int>(gi));
+          finish_groups(ready_groups(static_cast<int>(gi)));
           break;
         }
     }
+  }
+
+  // The graphs whose started group has finished its stream work (first,
+  // the graph `first` if it is one, already waited for).
+  std::vector<int> ready_groups(int first) {
+    std::vector<int> r;
+    if (first >= 0) r.push_back(first);
+    for (size_t gi = 0; gi < graphs.size(); ++gi) {
+      Graph& G = *graphs[gi];
+      if (!G.busy || static_cast<int>(gi) == first) continue;
+      const hipError_t e = hipEventQuery(G.ev);
+      if (e == hipSuccess) r.push_back(static_cast<int>(gi));
+      else if (e != hipErrorNotReady) CPG_HIP(e);
+    }
+    return r;
   }
 
   bool any_busy() const {

@@ -206,6 +206,7 @@ constexpr unsigned hipStreamNonBlocking = 1;
 const char* hipGetErrorString(hipError_t e);
 hipError_t hipGetLastError();
 hipError_t hipGetDevice(int* d);
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int dev);
 hipError_t simt_malloc(void** p, size_t bytes);
 template <class T>
