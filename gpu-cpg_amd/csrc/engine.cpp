@@ -1638,7 +1638,7 @@ struct cpg_server {
   // start together (one level-loop launch).  With nothing in flight,
   // admission waits for the first group in progress.
   void admit() {
-    finish_groups(ready_groups(-1));
+    finish_groups(ready_groups());
     std::vector<std::pair<int, std::vector<int64_t>>> batch;
     std::vector<uint8_t> in_batch(graphs.size(), 0);
     int held = 0;             // slots promised to the groups formed so far
@@ -1678,26 +1678,26 @@ struct cpg_server {
       batch.emplace_back(gi, std::move(ids));
     }
     if (!batch.empty()) start_groups(batch);
-    if (in_flight == 0) {
-      for (size_t gi = 0; gi < graphs.size(); ++gi)
-        if (graphs[gi]->busy) {
-          const double tw = now_ms();
-          CPG_HIP(hipEventSynchronize(graphs[gi]->ev));
-          stats.group_wait_ms += now_ms() - tw;
-          finish_groups(ready_groups(static_cast<int>(gi)));
-          break;
-        }
+    if (in_flight == 0 && any_busy()) {
+      // nothing to run meanwhile: wait for whichever groups finish first
+      // (the graphs' index builds end at different times)
+      const double tw = now_ms();
+      std::vector<int> r = ready_groups();
+      while (r.empty()) {
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        r = ready_groups();
+      }
+      stats.group_wait_ms += now_ms() - tw;
+      finish_groups(r);
     }
   }
 
-  // The graphs whose started group has finished its stream work (first,
-  // the graph `first` if it is one, already waited for).
-  std::vector<int> ready_groups(int first) {
+  // The graphs whose started group has finished its stream work.
+  std::vector<int> ready_groups() {
     std::vector<int> r;
-    if (first >= 0) r.push_back(first);
     for (size_t gi = 0; gi < graphs.size(); ++gi) {
       Graph& G = *graphs[gi];
-      if (!G.busy || static_cast<int>(gi) == first) continue;
+      if (!G.busy) continue;
       const hipError_t e = hipEventQuery(G.ev);
       if (e == hipSuccess) r.push_back(static_cast<int>(gi));
       else if (e != hipErrorNotReady) CPG_HIP(e);
