@@ -1528,11 +1528,15 @@ struct cpg_server {
   void finish_groups(const std::vector<int>& gis) {
     if (gis.empty()) return;
     std::vector<std::vector<unsigned>> maxd(gis.size());
-    for (size_t i = 0; i < gis.size(); ++i) maxd[i] = check_group(gis[i]);
+    std::vector<std::vector<Query*>> qp(gis.size());  // looked up here, not in the threads
+    for (size_t i = 0; i < gis.size(); ++i) {
+      maxd[i] = check_group(gis[i]);
+      for (int64_t id : graphs[gis[i]]->ids) qp[i].push_back(queries.at(id).get());
+    }
     auto run = [&](size_t i, std::mutex* mu) {
       Graph& G = *graphs[gis[i]];
       for (size_t q = 0; q < G.ids.size(); ++q) {
-        Query& qq = *queries[G.ids[q]];
+        Query& qq = *qp[i][q];
         float mad = 0.0f;
         std::memcpy(&mad, &maxd[i][q], 4);
         probe_query(qq, *G.h, G.args[q], mad);
