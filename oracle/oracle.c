@@ -495,6 +495,171 @@ done:
   return st->aborted ? -1 : got;
 }
 
+/* ------------------------------------------------------------------ */
+/* orc_topk with vertex sequences.  Every queued path keeps its         */
+/* deviation (from -> to) and the index of its popped parent, as the    */
+/* reference's PfxtNode keeps {from, to, parent} (gpucpg/gpucpg.cuh:     */
+/* 74-107; parent = node index at expansion, gpucpg.cu:3280-3288).  A   */
+/* path's vertices are its root's suffix chain with each deviation      */
+/* spliced in at its from-vertex: source, succ, ..., from, to, succ...  */
+/* (the chain walk of :2654-2694 read backwards).                       */
+/* ------------------------------------------------------------------ */
+typedef struct { float cost; int to; int from; int level; long long parent; } pq_path;
+typedef struct { pq_path* a; long long n, cap; } pathheap;
+static int ph_push(pathheap* h, pq_path x) {
+  if (h->n == h->cap) {
+    long long nc = h->cap ? h->cap * 2 : 1024;
+    pq_path* na = realloc(h->a, sizeof(pq_path) * (size_t)nc);
+    if (!na) return -1;
+    h->a = na; h->cap = nc;
+  }
+  long long i = h->n++;
+  while (i > 0) {
+    long long p = (i - 1) / 2;
+    if (h->a[p].cost <= x.cost) break;
+    h->a[i] = h->a[p];
+    i = p;
+  }
+  h->a[i] = x;
+  return 0;
+}
+static pq_path ph_pop(pathheap* h) {
+  pq_path top = h->a[0];
+  pq_path x = h->a[--h->n];
+  long long i = 0;
+  for (;;) {
+    long long l = 2 * i + 1, r = l + 1, s = i;
+    float sc = x.cost;
+    if (l < h->n && h->a[l].cost < sc) { s = l; sc = h->a[l].cost; }
+    if (r < h->n && h->a[r].cost < sc) { s = r; }
+    if (s == i) break;
+    h->a[i] = h->a[s];
+    i = s;
+  }
+  if (h->n > 0) h->a[i] = x;
+  return top;
+}
+
+typedef struct { float cost; long long pop; } cost_pop;
+static int cmp_cost_pop(const void* x, const void* y) {
+  const cost_pop* a = (const cost_pop*)x;
+  const cost_pop* b = (const cost_pop*)y;
+  if (a->cost != b->cost) return (a->cost > b->cost) - (a->cost < b->cost);
+  return (a->pop > b->pop) - (a->pop < b->pop);
+}
+
+/* Writes the vertices of popped path i of pops[] into out (cap entries);
+ * returns the count, or -1 when it does not fit or the chain is broken. */
+static long long path_vertices(const pq_path* pops, long long i, const int* succ, int n,
+                               int* devf, int* devt, int* out, long long cap) {
+  long long nd = 0;
+  long long r = i;
+  while (pops[r].from >= 0) {
+    if (nd >= n) return -1;
+    devf[nd] = pops[r].from;
+    devt[nd] = pops[r].to;
+    ++nd;
+    r = pops[r].parent;
+  }
+  int v = pops[r].to;
+  long long len = 0, di = nd;  /* deviations in root-to-leaf order: devf[nd-1] first */
+  while (v != -1) {
+    if (len >= cap || len > n) return -1;
+    out[len++] = v;
+    if (di > 0 && devf[di - 1] == v) {
+      v = devt[di - 1];
+      --di;
+    } else {
+      v = succ[v];
+    }
+  }
+  return di == 0 ? len : -1;
+}
+
+long long orc_topk_paths(const orc_graph* g, const int* dist, const int* succ, long long k,
+                         int root_tie_quirk, float* out_costs, long long* out_vptr,
+                         int* out_verts, long long vcap, orc_stats* st) {
+  orc_stats local;
+  if (!st) st = &local;
+  memset(st, 0, sizeof(*st));
+  if (out_vptr) out_vptr[0] = 0;
+  if (k <= 0) return 0;
+  const int n = g->n;
+  float* distf = malloc(sizeof(float) * (size_t)(n ? n : 1));
+  orc_distf(n, dist, distf);
+  pq_item* roots = NULL;
+  long long nr = make_roots(g, dist, root_tie_quirk, &roots, st);
+  pathheap pq = {0};
+  kheap kh = {malloc(sizeof(float) * (size_t)k), 0, k};
+  for (long long i = 0; i < nr; ++i) {
+    pq_path it = {roots[i].cost, roots[i].to, -1, 0, -1};
+    ph_push(&pq, it);
+    if (roots[i].cost != 0.0f) kh_offer(&kh, roots[i].cost);
+  }
+  free(roots);
+  long long npop = 0, pcap = 1024, got = 0;
+  pq_path* pops = malloc(sizeof(pq_path) * (size_t)pcap);
+  cost_pop* rep = malloc(sizeof(cost_pop) * (size_t)k);
+  while (got < k && pq.n > 0) {
+    const pq_path top = ph_pop(&pq);
+    if (npop == pcap) {
+      pcap *= 2;
+      pq_path* np = realloc(pops, sizeof(pq_path) * (size_t)pcap);
+      if (!np) { st->aborted = 1; goto done; }
+      pops = np;
+    }
+    const long long me = npop++;
+    pops[me] = top;
+    st->pops++;
+    if (top.cost == 0.0f) st->zero_cost++;
+    else { rep[got].cost = top.cost; rep[got].pop = me; ++got; }
+    const float slack = top.cost;
+    for (int v = top.to; v != -1; v = succ[v]) {
+      st->hops++;
+      const float dv = distf[v];
+      const int sv = succ[v];
+      for (int e = g->oadjp[v]; e < g->oadjp[v + 1]; ++e) {
+        st->edges++;
+        const int w = g->oadj[e];
+        if (w == sv) continue;
+        if (dist[w] == ORC_INT_MAX) continue;
+        st->children++;
+        const float c = slack + distf[w] + g->owgt[e] - dv;
+        if (c < slack) st->nonmonotone++;
+        if (kh.n == kh.k && c > kh.a[0]) continue;
+        pq_path it = {c, w, v, top.level + 1, me};
+        if (ph_push(&pq, it) != 0) { st->aborted = 1; goto done; }
+        st->pushes++;
+        if (c != 0.0f) kh_offer(&kh, c);
+      }
+    }
+  }
+  qsort(rep, (size_t)got, sizeof(cost_pop), cmp_cost_pop);
+  {
+    int* devf = malloc(sizeof(int) * (size_t)(n + 1));
+    int* devt = malloc(sizeof(int) * (size_t)(n + 1));
+    long long off = 0;
+    for (long long i = 0; i < got; ++i) {
+      out_costs[i] = rep[i].cost;
+      if (!out_vptr || !out_verts) continue;
+      const long long len = path_vertices(pops, rep[i].pop, succ, n, devf, devt, out_verts + off,
+                                          vcap - off);
+      if (len < 0) { st->aborted = 5; break; }
+      off += len;
+      out_vptr[i + 1] = off;
+    }
+    free(devf);
+    free(devt);
+  }
+done:
+  free(pq.a);
+  free(kh.a);
+  free(pops);
+  free(rep);
+  free(distf);
+  return st->aborted ? -1 : got;
+}
+
 /* Faithful SEQUENTIAL, gpucpg/gpucpg.cu:21230-21301 + :21332-21358:
  * every source seeds the queue (no finite-dist filter), k pops, every
  * deviation child pushed, final sort, zero-cost nodes erased. */
@@ -848,6 +1013,67 @@ long long orc_all_paths(const orc_graph* g, const int* dist,
   for (long long i = 0; i < c.n; ++i)
     if (out_costs[i] != 0.0f) out_costs[w++] = out_costs[i];
   return w;
+}
+
+/* Brute force with vertex sequences: the same DFS, keeping the vertex  */
+/* stack and writing each complete path's vertices (CSR in out_vptr).   */
+typedef struct {
+  dfs_ctx c;
+  int* stack;
+  long long depth;
+  long long* vptr;
+  int* verts;
+  long long vn, vcap;
+} dfsv_ctx;
+
+static int dfsv(dfsv_ctx* x, int u, float cost) {
+  x->stack[x->depth++] = u;
+  const int s = x->c.succ[u];
+  int rc = 0;
+  if (s == -1) {
+    if (x->c.n >= x->c.cap || x->vn + x->depth > x->vcap) { rc = -1; goto out; }
+    x->c.out[x->c.n] = cost;
+    memcpy(x->verts + x->vn, x->stack, sizeof(int) * (size_t)x->depth);
+    x->vn += x->depth;
+    x->vptr[++x->c.n] = x->vn;
+    goto out;
+  }
+  if (dfsv(x, s, cost) != 0) { rc = -1; goto out; }
+  for (int e = x->c.g->oadjp[u]; e < x->c.g->oadjp[u + 1]; ++e) {
+    const int w = x->c.g->oadj[e];
+    if (w == s || x->c.dist[w] == ORC_INT_MAX) continue;
+    const float nc = cost + x->c.distf[w] + x->c.g->owgt[e] - x->c.distf[u];
+    if (dfsv(x, w, nc) != 0) { rc = -1; goto out; }
+  }
+out:
+  x->depth--;
+  return rc;
+}
+
+/* Every source->sink path with its vertices, in DFS order (unsorted, zero
+ * costs included); returns the number of paths or -1 when a cap is hit. */
+long long orc_all_paths_vertices(const orc_graph* g, const int* dist, const int* succ,
+                                 long long cap, float* out_costs, long long* out_vptr,
+                                 int* out_verts, long long vcap) {
+  const int n = g->n;
+  float* distf = malloc(sizeof(float) * (size_t)(n ? n : 1));
+  orc_distf(n, dist, distf);
+  dfsv_ctx x;
+  memset(&x, 0, sizeof(x));
+  x.c.g = g; x.c.dist = dist; x.c.succ = succ; x.c.distf = distf;
+  x.c.out = out_costs; x.c.cap = cap;
+  x.stack = malloc(sizeof(int) * (size_t)(n + 1));
+  x.vptr = out_vptr; x.verts = out_verts; x.vcap = vcap;
+  out_vptr[0] = 0;
+  long long rc = 0;
+  for (int v = 0; v < n && rc == 0; ++v) {
+    if (g->iadjp[v + 1] - g->iadjp[v] != 0) continue;
+    if (dist[v] == ORC_INT_MAX) continue;
+    if (dfsv(&x, v, distf[v]) != 0) rc = -1;
+  }
+  free(distf);
+  free(x.stack);
+  return rc != 0 ? -1 : x.c.n;
 }
 
 /* ------------------------------------------------------------------ */

@@ -25,6 +25,13 @@
  *   orc_topk            brute-force enumeration of every source->sink path
  *                       (orc_all_paths) on tiny DAGs; the faithful SEQUENTIAL
  *                       and SHORT_LONG transliterations below on small DAGs.
+ *   orc_topk_paths      costs: orc_topk; vertex sequences: per cost class
+ *                       the same multiset as orc_all_paths_vertices (tiny
+ *                       DAGs and the reference benchmarks' small cases);
+ *                       each sequence a fanout-CSR path whose recurrence
+ *                       cost equals its reported cost.
+ *   orc_all_paths_vertices  every sequence is a source->sink walk of the CSR,
+ *                       their number equals a path count by DP over the DAG.
  *   orc_sequential      orc_all_paths (tiny DAGs).
  *   orc_short_long      orc_all_paths (tiny DAGs), orc_topk (small DAGs).
  *   orc_compare_prefix  unittests/tc_pfxt_inprocess.cu:26-51 (known answers).
@@ -137,6 +144,15 @@ long long orc_short_long(const orc_graph* g, const int* dist,
                          orc_stats* st);
 long long orc_all_paths(const orc_graph* g, const int* dist,
                         const int* succ, long long cap, float* out_costs);
+/* orc_topk that also reports each path's vertices (source ... sink),
+ * out_vptr[0..got] CSR offsets into out_verts (vcap entries). */
+long long orc_topk_paths(const orc_graph* g, const int* dist, const int* succ, long long k,
+                         int root_tie_quirk, float* out_costs, long long* out_vptr,
+                         int* out_verts, long long vcap, orc_stats* st);
+/* Brute force with vertices: every path (DFS order, zero costs kept). */
+long long orc_all_paths_vertices(const orc_graph* g, const int* dist, const int* succ,
+                                 long long cap, float* out_costs, long long* out_vptr,
+                                 int* out_verts, long long vcap);
 
 /* ---- compare (gate semantics) ---- */
 typedef struct {

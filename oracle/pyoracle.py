@@ -64,6 +64,9 @@ _SIGS = {
     "orc_short_long": (_LL, [_PG, _PI, _PI, _LL, ctypes.c_float, ctypes.c_int, ctypes.c_float,
                              ctypes.c_double, _LL, _PF, _PS]),
     "orc_all_paths": (_LL, [_PG, _PI, _PI, _LL, _PF]),
+    "orc_topk_paths": (_LL, [_PG, _PI, _PI, _LL, ctypes.c_int, _PF, ctypes.POINTER(_LL), _PI,
+                             _LL, _PS]),
+    "orc_all_paths_vertices": (_LL, [_PG, _PI, _PI, _LL, _PF, ctypes.POINTER(_LL), _PI, _LL]),
     "orc_compare_prefix": (None, [_PF, _LL, _PF, _LL, ctypes.c_int, ctypes.c_float,
                                   ctypes.POINTER(Compare)]),
     "orc_first_bit_mismatch": (_LL, [_PF, _PF, _LL]),
@@ -205,6 +208,37 @@ class OracleGraph:
         if got < 0:
             raise ShortLongAbort(st.as_dict())
         return out[:got], st.as_dict()
+
+    def topk_paths(self, dist, succ, k, root_tie_quirk=True, max_len=None):
+        """orc_topk with each path's vertex sequence: (costs, [vertex arrays], stats)."""
+        k = int(k)
+        if max_len is None:
+            max_len = self.g.n + 1
+        vcap = max(1, k * int(max_len))
+        out = np.zeros(max(k, 1), np.float32)
+        vptr = np.zeros(k + 1, np.int64)
+        verts = np.zeros(vcap, np.int32)
+        st = Stats()
+        got = lib().orc_topk_paths(ctypes.byref(self.g), _ip(dist), _ip(succ), k,
+                                   1 if root_tie_quirk else 0, _fp(out),
+                                   vptr.ctypes.data_as(ctypes.POINTER(_LL)), _ip(verts), vcap,
+                                   ctypes.byref(st))
+        if got < 0:
+            raise RuntimeError("oracle top-k paths aborted: %r" % st.as_dict())
+        paths = [verts[vptr[i]:vptr[i + 1]].copy() for i in range(got)]
+        return out[:got], paths, st.as_dict()
+
+    def all_paths_vertices(self, dist, succ, cap=100_000, vcap=4_000_000):
+        """Brute force: every path's (cost, vertices), DFS order, zero costs kept."""
+        out = np.zeros(cap, np.float32)
+        vptr = np.zeros(cap + 1, np.int64)
+        verts = np.zeros(vcap, np.int32)
+        got = lib().orc_all_paths_vertices(ctypes.byref(self.g), _ip(dist), _ip(succ), int(cap),
+                                           _fp(out), vptr.ctypes.data_as(ctypes.POINTER(_LL)),
+                                           _ip(verts), int(vcap))
+        if got < 0:
+            raise RuntimeError("too many paths for brute force")
+        return out[:got], [verts[vptr[i]:vptr[i + 1]].copy() for i in range(got)]
 
     def all_paths(self, dist, succ, cap=1_000_000):
         out = np.zeros(cap, np.float32)

@@ -448,3 +448,130 @@ def test_zero_cost_paths_are_erased():
     assert st["zero_cost"] == 2
     seq, _ = g.sequential(dist, succ, 10)
     assert seq.tolist() == [1.0]
+
+
+# ------------------------------------------- vertex sequences (f1 row 1)
+def py_edge_paths(n, adjp, adj, wgt, iadjp, dist, succ):
+    """Every source->sink walk of the fanout CSR, edge by edge (parallel
+    edges are distinct walks), reduced to the prefix-tree's path identity:
+    a step onto succ[u] is on the suffix chain whatever edge it uses, a step
+    to any other w (finite dist) is a deviation through that edge.  Returns
+    {(cost bits, vertex tuple, deviation edge tuple)}: each entry one path
+    of the reference's enumeration, with its recurrence cost."""
+    distf = (np.asarray(dist, np.int64).astype(np.int32).astype(F32) / F32(10000)).astype(F32)
+    found = set()
+
+    def walk(u, c, verts, devs):
+        if adjp[u + 1] == adjp[u]:
+            if dist[u] != INT_MAX:
+                found.add((int(np.float32(c).view(np.uint32)), tuple(verts), tuple(devs)))
+            return
+        for e in range(adjp[u], adjp[u + 1]):
+            w = int(adj[e])
+            if dist[w] == INT_MAX:
+                continue
+            if w == succ[u]:
+                walk(w, c, verts + [w], devs)
+            else:
+                walk(w, F32(F32(F32(c + distf[w]) + wgt[e]) - distf[u]), verts + [w], devs + [e])
+    for v in range(n):
+        if iadjp[v + 1] == iadjp[v] and dist[v] != INT_MAX:
+            walk(v, distf[v], [v], [])
+    return found
+
+
+def class_multiset(costs, paths):
+    from collections import Counter
+    return Counter((int(np.float32(c).view(np.uint32)), tuple(int(x) for x in p))
+                   for c, p in zip(costs, paths))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_topk_paths_equal_edge_walk_brute_force(seed):
+    from collections import Counter
+    n = 14
+    wset = [0.1, 0.2, 0.25, 0.5, 1.0] if seed % 2 else None
+    edges = random_dag(300 + seed, n, 30, wset=wset)
+    g = og_from(n, edges)
+    adjp, adj, wgt, iadjp, iadj = csr_from_edges(n, edges)
+    dist, succ, _, _ = g.sfxt()
+    brute_set = py_edge_paths(n, adjp, adj, wgt, iadjp, dist.astype(np.int64), succ)
+    nonzero = [(cb, vs) for cb, vs, _ in brute_set if cb not in (0, 0x80000000)]
+    want_all = Counter(nonzero)
+    assert len(nonzero) > 0
+    costs, paths, st = g.topk_paths(dist, succ, len(nonzero), root_tie_quirk=False)
+    # every nonzero path, each with its recurrence cost and vertices
+    assert class_multiset(costs, paths) == want_all
+    ref_costs, _ = g.topk(dist, succ, len(nonzero), root_tie_quirk=False)
+    np.testing.assert_array_equal(costs, ref_costs)
+    for k in (1, 3, 7):
+        if k > len(nonzero):
+            continue
+        ck, pk, _ = g.topk_paths(dist, succ, k, root_tie_quirk=False)
+        np.testing.assert_array_equal(ck, ref_costs[:k])
+        got = class_multiset(ck, pk)
+        for key, cnt in got.items():
+            assert want_all[key] >= cnt
+        # cost classes strictly below the k-th cost are complete
+        kth = int(np.float32(ck[-1]).view(np.uint32))
+        below = Counter({key: c for key, c in want_all.items()
+                         if np.uint32(key[0]).view(np.float32) < ck[-1]})
+        assert Counter({key: c for key, c in got.items() if key[0] != kth}) == below
+
+
+def test_topk_paths_fixtures_brute_force(golden_dir):
+    for name in ("simple.txt", "dac21_paper_example.txt"):
+        g = po.OracleGraph.from_text(fixture(golden_dir, name))
+        a = g.arrays()
+        dist, succ, _, _ = g.sfxt()
+        brute_set = py_edge_paths(g.n, a["oadjp"], a["oadj"], a["owgt"], a["iadjp"],
+                                  dist.astype(np.int64), succ)
+        from collections import Counter
+        want = Counter((cb, vs) for cb, vs, _ in brute_set if cb not in (0, 0x80000000))
+        costs, paths, _ = g.topk_paths(dist, succ, len(want) + 3)
+        assert class_multiset(costs, paths) == want
+    # dac21: the cheapest path is 0 -> 4 -> 7 -> 10 (cost 1 + 4 + 2)
+    g = po.OracleGraph.from_text(fixture(golden_dir, "dac21_paper_example.txt"))
+    dist, succ, _, _ = g.sfxt()
+    costs, paths, _ = g.topk_paths(dist, succ, 1)
+    assert costs[0] == 7.0 and paths[0].tolist() == [0, 4, 7, 10]
+
+
+def test_topk_paths_through_zero_cost_parent():
+    # the only nonzero path 0 -> 3 -> 1 is a child of the zero-cost root 0 -> 1
+    edges = [(0, 1, 0.0), (0, 3, 0.5), (3, 1, 0.5)]
+    g = og_from(4, edges)
+    dist, succ, _, _ = g.sfxt()
+    costs, paths, st = g.topk_paths(dist, succ, 10, root_tie_quirk=False)
+    assert costs.tolist() == [1.0]
+    assert [p.tolist() for p in paths] == [[0, 3, 1]]
+    assert st["zero_cost"] == 2
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_all_paths_vertices_brute_force(seed):
+    """The C brute force with vertices: the same (cost, vertices) multiset
+    as the edge-walk enumeration, and as many paths as the path-count DP
+    P(v) = 1 at a chain end, else P(succ v) + sum of P(w) over deviations."""
+    from collections import Counter
+    n = 13
+    edges = random_dag(500 + seed, n, 28)
+    g = og_from(n, edges)
+    adjp, adj, wgt, iadjp, iadj = csr_from_edges(n, edges)
+    dist, succ, _, _ = g.sfxt()
+    costs, paths = g.all_paths_vertices(dist, succ)
+    brute_set = py_edge_paths(n, adjp, adj, wgt, iadjp, dist.astype(np.int64), succ)
+    assert class_multiset(costs, paths) == Counter((cb, vs) for cb, vs, _ in brute_set)
+    memo = {}
+
+    def count(v):
+        if v not in memo:
+            s = int(succ[v])
+            if s == -1:
+                memo[v] = 1
+            else:
+                memo[v] = count(s) + sum(count(int(adj[e])) for e in range(adjp[v], adjp[v + 1])
+                                         if int(adj[e]) != s and dist[int(adj[e])] != INT_MAX)
+        return memo[v]
+    total = sum(count(v) for v in range(n) if iadjp[v + 1] == iadjp[v] and dist[v] != INT_MAX)
+    assert len(costs) == total
