@@ -10,6 +10,7 @@
 #include <fstream>
 #include <random>
 #include <stdexcept>
+#include <thread>
 
 namespace cpg {
 
@@ -75,7 +76,76 @@ static bool parse_int(const char* b, const char* e, int* out) {
   return true;
 }
 
-void parse_text(const std::string& path, bool ignore_wgts, EdgeList& el) {
+namespace {
+
+// Parses the edge lines in [b, e) (whole lines) into u/v/w, in order.  On a
+// malformed line sets err and the line's index within the range.
+struct EdgeChunk {
+  std::vector<int> u, v;
+  std::vector<float> w;
+  std::string err;
+  int64_t err_line = -1;  // line index within the chunk
+};
+
+void parse_edge_lines(const char* b, const char* e, bool ignore_wgts, EdgeChunk& out) {
+  const char* p = b;
+  int64_t li = 0;
+  for (; p < e; ++li) {
+    const char* lb = p;
+    const char* nl = static_cast<const char*>(std::memchr(p, '\n', static_cast<size_t>(e - p)));
+    const char* le = nl ? nl : e;
+    p = nl ? nl + 1 : e;
+    const char* q[4];
+    int nq = 0;
+    for (const char* c = lb; c < le && nq < 4; ++c)
+      if (*c == '"') q[nq++] = c;
+    int u, v;
+    if (nq < 4 || !parse_int(q[0] + 1, q[1], &u) || !parse_int(q[2] + 1, q[3], &v)) {
+      out.err = "malformed edge";
+      out.err_line = li;
+      return;
+    }
+    float w = 1.0f;
+    if (!ignore_wgts && std::memchr(lb, ',', static_cast<size_t>(le - lb)) != nullptr) {
+      const char* c = static_cast<const char*>(std::memchr(q[3] + 1, ',', static_cast<size_t>(le - q[3] - 1)));
+      const char* s = c ? c + 1 : nullptr;
+      const char* semi = s ? static_cast<const char*>(std::memchr(s, ';', static_cast<size_t>(le - s))) : nullptr;
+      const char* we = semi ? semi : le;
+      char wb[128];
+      const size_t wl = s ? static_cast<size_t>(we - s) : 0;
+      char* e2 = nullptr;
+      if (s && wl < sizeof(wb)) {
+        std::memcpy(wb, s, wl);
+        wb[wl] = 0;
+        w = std::strtof(wb, &e2);
+      }
+      if (!s || wl >= sizeof(wb) || e2 == wb) {
+        out.err = "malformed weight";
+        out.err_line = li;
+        return;
+      }
+    }
+    out.u.push_back(u);
+    out.v.push_back(v);
+    out.w.push_back(w);
+  }
+}
+
+int64_t count_lines(const char* b, const char* e) {
+  int64_t c = 0;
+  for (const char* p = b; p < e; ++c) {
+    const char* nl = static_cast<const char*>(std::memchr(p, '\n', static_cast<size_t>(e - p)));
+    p = nl ? nl + 1 : e;
+  }
+  return c;
+}
+
+}  // namespace
+
+// The edge lines are cut into chunks at line breaks and parsed on up to 16
+// host threads; the chunks are joined in file order, so the edge order (and
+// hence the CSR order) is the file's.
+void parse_text(const std::string& path, bool ignore_wgts, EdgeList& el, int threads) {
   std::FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) throw std::runtime_error("Unable to open file: " + path);
   std::fseek(f, 0, SEEK_END);
@@ -103,35 +173,90 @@ void parse_text(const std::string& path, bool ignore_wgts, EdgeList& el) {
   el.n = n;
   for (int i = 0; i < n; ++i)
     if (!next_line(lb, le)) break;
-  int64_t lineno = n + 1;
-  while (next_line(lb, le)) {
-    ++lineno;
-    const char* q[4];
-    int nq = 0;
-    for (const char* c = lb; c < le && nq < 4; ++c)
-      if (*c == '"') q[nq++] = c;
-    int u, v;
-    if (nq < 4 || !parse_int(q[0] + 1, q[1], &u) || !parse_int(q[2] + 1, q[3], &v))
-      throw std::runtime_error("malformed edge at line " + std::to_string(lineno));
-    float w = 1.0f;
-    if (!ignore_wgts && std::memchr(lb, ',', static_cast<size_t>(le - lb)) != nullptr) {
-      const char* c = static_cast<const char*>(std::memchr(q[3] + 1, ',', static_cast<size_t>(le - q[3] - 1)));
-      if (!c) throw std::runtime_error("malformed weight at line " + std::to_string(lineno));
-      const char* s = c + 1;
-      const char* semi = static_cast<const char*>(std::memchr(s, ';', static_cast<size_t>(le - s)));
-      const char* we = semi ? semi : le;
-      char wb[128];
-      const size_t wl = static_cast<size_t>(we - s);
-      if (wl >= sizeof(wb)) throw std::runtime_error("malformed weight at line " + std::to_string(lineno));
-      std::memcpy(wb, s, wl);
-      wb[wl] = 0;
-      char* e2 = nullptr;
-      w = std::strtof(wb, &e2);
-      if (e2 == wb) throw std::runtime_error("malformed weight at line " + std::to_string(lineno));
+  const int64_t first_line = static_cast<int64_t>(n) + 2;  // 1-based line of the first edge
+  const size_t bytes = static_cast<size_t>(end - p);
+  int nt = threads > 0 ? threads : static_cast<int>(std::thread::hardware_concurrency());
+  nt = std::max(1, std::min({nt, 16, static_cast<int>(bytes / (1u << 20)) + 1}));
+  std::vector<const char*> cut(static_cast<size_t>(nt) + 1, end);
+  cut[0] = p;
+  for (int t = 1; t < nt; ++t) {
+    const char* c = p + bytes * static_cast<size_t>(t) / static_cast<size_t>(nt);
+    if (c < cut[t - 1]) c = cut[t - 1];
+    const char* nl = c < end ? static_cast<const char*>(std::memchr(c, '\n', static_cast<size_t>(end - c))) : nullptr;
+    cut[t] = nl ? nl + 1 : end;
+  }
+  std::vector<EdgeChunk> ch(static_cast<size_t>(nt));
+  if (nt == 1) {
+    parse_edge_lines(cut[0], cut[1], ignore_wgts, ch[0]);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&, t]() { parse_edge_lines(cut[t], cut[t + 1], ignore_wgts, ch[t]); });
+    for (auto& x : th) x.join();
+  }
+  for (int t = 0; t < nt; ++t)
+    if (!ch[t].err.empty()) {
+      const int64_t line = first_line + count_lines(cut[0], cut[t]) + ch[t].err_line;
+      throw std::runtime_error(ch[t].err + " at line " + std::to_string(line));
     }
-    el.u.push_back(u);
-    el.v.push_back(v);
-    el.w.push_back(w);
+  size_t m = 0;
+  for (const auto& c : ch) m += c.u.size();
+  el.u.reserve(m);
+  el.v.reserve(m);
+  el.w.reserve(m);
+  for (auto& c : ch) {
+    el.u.insert(el.u.end(), c.u.begin(), c.u.end());
+    el.v.insert(el.v.end(), c.v.begin(), c.v.end());
+    el.w.insert(el.w.end(), c.w.begin(), c.w.end());
+  }
+}
+
+// Binary CSR (gpucpg/graph.h:24-55): N, M, offsets[N+1], adjncy[M],
+// wgts[M], each a native-endian 4-byte word.
+void write_bin(const HostGraph& g, const std::string& path) {
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("Unable to open file: " + path);
+  const int32_t hdr[2] = {g.n, static_cast<int32_t>(g.m)};
+  bool ok = std::fwrite(hdr, 4, 2, f) == 2;
+  ok = ok && std::fwrite(g.oadjp.data(), 4, g.oadjp.size(), f) == g.oadjp.size();
+  ok = ok && std::fwrite(g.oadj.data(), 4, g.oadj.size(), f) == g.oadj.size();
+  ok = ok && std::fwrite(g.owgt.data(), 4, g.owgt.size(), f) == g.owgt.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) throw std::runtime_error("write failed: " + path);
+}
+
+void read_bin(const std::string& path, EdgeList& el) {
+  std::FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw std::runtime_error("Unable to open file: " + path);
+  std::fseek(f, 0, SEEK_END);
+  const long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  int32_t hdr[2] = {0, 0};
+  if (sz < 8 || std::fread(hdr, 4, 2, f) != 2) {
+    std::fclose(f);
+    throw std::runtime_error("binary CSR too short: " + path);
+  }
+  const int64_t n = hdr[0], m = hdr[1];
+  if (n < 0 || m < 0 || 8 + 4 * (n + 1) + 8 * m != static_cast<int64_t>(sz)) {
+    std::fclose(f);
+    throw std::runtime_error("binary CSR size does not match its header: " + path);
+  }
+  std::vector<int> adjp(static_cast<size_t>(n) + 1);
+  el = EdgeList();
+  el.n = static_cast<int>(n);
+  el.v.resize(static_cast<size_t>(m));
+  el.w.resize(static_cast<size_t>(m));
+  bool ok = std::fread(adjp.data(), 4, adjp.size(), f) == adjp.size();
+  ok = ok && std::fread(el.v.data(), 4, el.v.size(), f) == el.v.size();
+  ok = ok && std::fread(el.w.data(), 4, el.w.size(), f) == el.w.size();
+  std::fclose(f);
+  if (!ok) throw std::runtime_error("binary CSR read failed: " + path);
+  if (adjp[0] != 0 || adjp[static_cast<size_t>(n)] != m)
+    throw std::runtime_error("binary CSR offsets do not span the edges");
+  el.u.resize(static_cast<size_t>(m));
+  for (int64_t u = 0; u < n; ++u) {
+    if (adjp[u + 1] < adjp[u]) throw std::runtime_error("binary CSR offsets decrease");
+    for (int e = adjp[u]; e < adjp[u + 1]; ++e) el.u[e] = static_cast<int>(u);
   }
 }
 

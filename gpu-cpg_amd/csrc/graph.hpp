@@ -40,8 +40,15 @@ void build_csr(const EdgeList& el, HostGraph& g);
 //   then one edge per line:  "u" -> "v"[, w];   (default weight 1.0f,
 //   ignore_wgts forces 1.0f).  Throws std::runtime_error on a malformed
 //   line, matching the reference's std::stoi / std::stof failures.
-void parse_text(const std::string& path, bool ignore_wgts, EdgeList& el);
+//   The edge lines are parsed on up to 16 host threads (threads = 0: one per
+//   hardware thread), joined in file order.
+void parse_text(const std::string& path, bool ignore_wgts, EdgeList& el, int threads = 0);
 void write_text(const EdgeList& el, const std::string& path);
+// Binary fanout CSR, gpucpg/graph.h:24-55: int32 N, int32 M, offsets[N+1],
+// adjncy[M], float wgts[M] (native-endian).  read_bin checks the size and
+// the offsets and returns the edges in CSR order.
+void write_bin(const HostGraph& g, const std::string& path);
+void read_bin(const std::string& path, EdgeList& el);
 
 // Seeded layered DAG generator.
 struct GenParams {
