@@ -13,7 +13,9 @@
 //   get_pfxt_nodes      gpucpg/gpucpg.cuh:184, gpucpg/gpucpg.cu:21422-21440
 //   num_verts/num_edges gpucpg/gpucpg.cuh:208-209
 //   static cache        gpucpg/gpucpg.cuh:186-189
-//   timing members      gpucpg/gpucpg.cuh:327-356
+//   timing members      gpucpg/gpucpg.cuh:327-356 (short_long_step_times and
+//                       paths_gen_per_step: one entry per expansion round)
+//   write_to_csr_bin    gpucpg/gpucpg.cuh:310-323, binary layout gpucpg/graph.h:24-55
 // Errors throw std::runtime_error with the library's message, as the
 // reference throws on I/O and capacity failures (device errors there end
 // the process; here they are reported the same way as other errors).
@@ -21,6 +23,8 @@
 #define GPUCPG_AMD_HPP
 
 #include <chrono>
+#include <cmath>
+#include <cstdint>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -92,7 +96,7 @@ class CpGen {
                     const float init_split_perc = 0.005f, const float /*alpha*/ = 5.0f,
                     const int /*per_thread_work_items*/ = 8, bool /*enable_reindex_cpu*/ = false,
                     bool /*enable_reindex_gpu*/ = false, bool /*enable_fuse_steps*/ = false,
-                    bool /*enable_interm_perf_log*/ = false,
+                    bool enable_interm_perf_log = false,
                     const CsrReorderMethod /*csr_reorder_method*/ = CsrReorderMethod::E_ORIENTED,
                     bool /*enable_tile_spur*/ = false,
                     std::optional<float> /*fixed_split_inc_amount*/ = std::nullopt,
@@ -100,14 +104,48 @@ class CpGen {
     cpg_query_opts o;
     cpg_query_opts_default(&o);
     o.init_split_perc = init_split_perc;
+    o.step_times = 1;
+    o.kernel_timing = enable_interm_perf_log ? 1 : 0;
     check(cpg_query(h_, k, &o));
     cpg_timing t{};
     check(cpg_timings(h_, &t));
     prop_time = usec(t.sfxt_ms * 1e3);
     expand_time = usec(t.pfxt_ms * 1e3);
+    // levelize, relax and successor run as one pass over the levels: its
+    // device time is lvlize_time (with enable_interm_perf_log) and relax_time
+    // stays 0, so lvlize_time + relax_time is the time of both
+    lvlize_time = usec(t.sfxt_levels_ms * 1e3);
+    relax_time = usec(0.0);
     short_long_expansion_steps = static_cast<int>(t.rounds);
     total_gen_paths = static_cast<int>(t.generated);
     graph_diameter = cpg_graph_diameter(h_);
+    const int64_t nr = cpg_step_times(h_, nullptr, nullptr, 0);
+    check(nr < 0 ? -1 : 0);
+    std::vector<double> ms(static_cast<size_t>(nr));
+    std::vector<int64_t> paths(static_cast<size_t>(nr));
+    cpg_step_times(h_, ms.data(), paths.data(), nr);
+    short_long_step_times.clear();
+    paths_gen_per_step.clear();
+    for (int64_t i = 0; i < nr; ++i) {
+      short_long_step_times.push_back(usec(ms[static_cast<size_t>(i)] * 1e3));
+      paths_gen_per_step.push_back(static_cast<int>(paths[static_cast<size_t>(i)]));
+    }
+  }
+
+  // gpucpg/gpucpg.cuh:292-299 (initial split increment of the reference's
+  // schedule; the engine's exact rounds do not use it)
+  float compute_split_inc_amount(float avg_deg) {
+    const float lo = 0.1f, hi = 100.0f, d0 = 3.0f, kk = 0.8f;
+    return lo + (hi - lo) / (1 + std::exp(kk * (avg_deg - d0)));
+  }
+
+  // gpucpg/gpucpg.cuh:310-323 with graph.h:24-38: the fanout CSR as
+  // N, M, offsets[N+1], adjncy[M], wgts[M] (native-endian 4-byte words)
+  void write_to_csr_bin(std::string& filename) { check(cpg_write_bin(h_, filename.c_str())); }
+  // The same format back in (graph.h:40-55), as read_input does for text.
+  void read_input_bin(const std::string& filename) {
+    check(cpg_load_bin(h_, filename.c_str()));
+    benchmark_path = filename;
   }
 
   std::vector<float> get_slacks(int k) {
@@ -156,11 +194,27 @@ class CpGen {
   }
   cpg_handle* handle() const { return h_; }
 
+  // timing members, gpucpg/gpucpg.cuh:327-356
   usec prop_time{0};
   usec expand_time{0};
+  usec lvlize_time{0};
+  usec prefix_scan_time{0};
+  usec csr_reorder_time{0};
+  usec relax_time{0};
+  usec rabbit_copy_to_gpu_time{0};
+  usec rabbit_update_csr_time{0};
+  usec gorder_copy_to_gpu_time{0};
+  usec gorder_update_csr_time{0};
+  usec corder_copy_to_gpu_time{0};
+  usec corder_update_csr_time{0};
+  usec gpba_transfer_time{0};
+  usec gpba_pfxt_expand_time{0};
+  usec gpba_sort_and_prune_time{0};
   int short_long_expansion_steps{0};
   std::string benchmark_path;
   int graph_diameter{0};
+  std::vector<int> paths_gen_per_step;
+  std::vector<usec> short_long_step_times;
   int total_gen_paths{0};
 
  private:
