@@ -27,7 +27,8 @@ class QueryOpts(ctypes.Structure):
                 ("max_nodes", ctypes.c_int64), ("mem_budget_bytes", ctypes.c_int64),
                 ("endpoint_mask", ctypes.c_void_p), ("collect_stats", ctypes.c_int),
                 ("kernel_timing", ctypes.c_int), ("sfxt_blocks", ctypes.c_int),
-                ("bound_probe", ctypes.c_int)]
+                ("bound_probe", ctypes.c_int), ("sfxt_method", ctypes.c_int),
+                ("step_times", ctypes.c_int)]
 
 
 class Timing(ctypes.Structure):
@@ -75,6 +76,8 @@ ABI = {
     "cpg_last_error": (ctypes.c_char_p, []),
     "cpg_load_text": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
     "cpg_load_edges": (ctypes.c_int, [_H, ctypes.c_int, _I64, _PI, _PI, _PF]),
+    "cpg_load_bin": (ctypes.c_int, [_H, ctypes.c_char_p]),
+    "cpg_write_bin": (ctypes.c_int, [_H, ctypes.c_char_p]),
     "cpg_load_csr": (ctypes.c_int, [_H, ctypes.c_int, _I64, _PI, _PI, _PF]),
     "cpg_get_csr": (ctypes.c_int, [_H, _PI, _PI, _PF]),
     "cpg_num_verts": (_I64, [_H]),
@@ -88,6 +91,7 @@ ABI = {
     "cpg_get_nodes": (_I64, [_H, _I64, ctypes.POINTER(CpgNode)]),
     "cpg_get_path_vertices": (_I64, [_H, _I64, _PI, _I64]),
     "cpg_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
+    "cpg_step_times": (_I64, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64), _I64]),
     "cpg_sfxt": (ctypes.c_int, [_H]),
     "cpg_get_sfxt": (ctypes.c_int, [_H, _PI, _PI, _PI]),
     "cpg_batch_create": (_H, []),
@@ -224,6 +228,13 @@ class CpGen:
     def read_input(self, filename, ignore_wgts=False):
         _c(self, self.L.cpg_load_text(self.h, filename.encode(), 1 if ignore_wgts else 0))
 
+    def read_input_bin(self, filename):
+        """Binary fanout CSR (gpucpg/graph.h:24-55)."""
+        _c(self, self.L.cpg_load_bin(self.h, filename.encode()))
+
+    def write_to_csr_bin(self, filename):
+        _c(self, self.L.cpg_write_bin(self.h, filename.encode()))
+
     def load_edges(self, n, u, v, w):
         u = np.ascontiguousarray(u, np.int32)
         v = np.ascontiguousarray(v, np.int32)
@@ -261,7 +272,8 @@ class CpGen:
 
     def report_paths(self, k, *args, root_tie_quirk=True, batch=0, max_nodes=0,
                      collect_stats=False, init_split_perc=0.005, kernel_timing=False,
-                     sfxt_blocks=0, bound_probe=0, endpoint_mask=None, **kwargs):
+                     sfxt_blocks=0, bound_probe=0, endpoint_mask=None, sfxt_method=0,
+                     step_times=False, **kwargs):
         """Runs the GPG query for the k lowest-cost paths.  Positional
         arguments after k (max_dev_lvls, enable_compress, methods, ...) are
         accepted for call compatibility with the reference; the engine always
@@ -278,6 +290,8 @@ class CpGen:
         o.kernel_timing = 1 if kernel_timing else 0
         o.sfxt_blocks = int(sfxt_blocks)
         o.bound_probe = int(bound_probe)
+        o.sfxt_method = int(sfxt_method)
+        o.step_times = 1 if step_times else 0
         mask = None
         if endpoint_mask is not None:
             mask = np.ascontiguousarray(endpoint_mask, dtype=np.uint8)
@@ -318,6 +332,15 @@ class CpGen:
         t = Timing()
         _c(self, self.L.cpg_timings(self.h, ctypes.byref(t)))
         return t.as_dict()
+
+    def step_times(self):
+        """(per-round device ms, per-round stored paths) of the last query
+        run with step_times=True."""
+        n = _c(self, self.L.cpg_step_times(self.h, None, None, 0))
+        ms = (ctypes.c_double * max(n, 1))()
+        paths = (_I64 * max(n, 1))()
+        n = _c(self, self.L.cpg_step_times(self.h, ms, paths, n))
+        return [ms[i] for i in range(n)], [paths[i] for i in range(n)]
 
     def sfxt(self):
         n = self.num_verts()
@@ -412,12 +435,13 @@ class Batch:
         return _check(self.L.cpg_batch_add_edges(self.h, int(n), len(u), _iptr(u), _iptr(v),
                                                  _fptr(w), int(k)), self.L)
 
-    def run(self, root_tie_quirk=True, init_split_perc=0.005, sfxt_blocks=0):
+    def run(self, root_tie_quirk=True, init_split_perc=0.005, sfxt_blocks=0, sfxt_method=0):
         o = QueryOpts()
         self.L.cpg_query_opts_default(ctypes.byref(o))
         o.root_tie_quirk = 1 if root_tie_quirk else 0
         o.init_split_perc = float(init_split_perc)
         o.sfxt_blocks = int(sfxt_blocks)
+        o.sfxt_method = int(sfxt_method)
         _check(self.L.cpg_batch_run(self.h, ctypes.byref(o)), self.L)
 
     def get_slacks(self, q, k):

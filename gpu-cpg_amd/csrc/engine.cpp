@@ -160,11 +160,13 @@ struct cpg_handle {
   DevBuf<int4> vrec, devs, devs_tmp;
   DevBuf<int2> devinfo, lift;
   DevBuf<SfxtCtl> sctl;
+  DevBuf<unsigned long long> dfw;  // dataflow loop: one progress word per block
   bool sfxt_valid = false;
   std::vector<uint8_t> sfxt_sinks;  // endpoint subset of the current SFXT, per sink (empty = all)
   DevBuf<uint8_t> emask;
   int diameter = 0;
   int sfxt_blocks = 0;       // level-loop grid requested (0 = automatic)
+  int sfxt_method = 0;       // 0 dataflow loop, 1 level-synchronous loop
   int last_diameter = 0;     // levels of the last SFXT on this graph (automatic grid)
   float max_abs_distf = 0.0f;
   bool cache_enabled = false;
@@ -188,11 +190,17 @@ struct cpg_handle {
   DevBuf<unsigned long long> nsel;
   DevBuf<float> r_cost;
   DevBuf<int> r_level, r_from, r_to, r_pidx, r_parent, rank_of;
+  DevBuf<uint32_t> r_node;   // node-store slot of each reported path
   int64_t nres = 0;
+  uint64_t nodes_stored = 0; // node-store slots the last query filled
   bool have_result = false;
   std::vector<float> h_cost;
   std::vector<int> h_level, h_from, h_to, h_parent;
   bool host_result = false;
+  // host copies of the prefix tree for vertex reports (filled on first use)
+  std::vector<uint32_t> h_node;
+  std::vector<int> hn_from, hn_to, hn_parent;
+  bool host_tree = false;
 
   // BVSS deviation index (host copy for decoding, device copy for discovery)
   struct HostBvss {
@@ -214,10 +222,27 @@ struct cpg_handle {
   cpg_timing timing{};
   bool kernel_timing = false;
   EventPair ev_sfxt, ev_levels, ev_count, ev_fill;
+  // per-round record (step_times option): an event after every round and
+  // the stored-path count the round ended with
+  static constexpr int kRoundLog = 4096;
+  std::vector<hipEvent_t> rev;
+  DevBuf<unsigned long long> rlog;
+  std::vector<double> step_ms;
+  std::vector<int64_t> step_paths;
 
   cpg_handle() {}
   ~cpg_handle() {
+    for (hipEvent_t e : rev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  hipEvent_t round_event(size_t i) {
+    while (rev.size() <= i) {
+      hipEvent_t e = nullptr;
+      CPG_HIP(hipEventCreate(&e));
+      rev.push_back(e);
+    }
+    return rev[i];
   }
 
   void ensure_stream() {
@@ -250,6 +275,7 @@ struct cpg_handle {
   void reset() {  // CpGen::reset, gpucpg/gpucpg.cuh:211-250
     have_result = false;
     host_result = false;
+    host_tree = false;
     nres = 0;
     timing = cpg_timing{};
     if (!cache_enabled) invalidate_cache_payload();
@@ -344,14 +370,24 @@ struct cpg_handle {
         h2d(emask.p, mask, n, stream);
         a.sink_mask = emask.p;
       }
+      const bool dataflow = sfxt_method == 0;
+      int df_grid = 0;
+      if (dataflow) {
+        df_grid = sfxt_dataflow_grid(sfxt_blocks);
+        dfw.reserve(static_cast<size_t>(df_grid));
+        a.dfw = dfw.p;
+        a.dfw_n = df_grid;
+      }
       if (kernel_timing) {
         ev_sfxt.create();
         ev_levels.create();
         CPG_HIP(hipEventRecord(ev_sfxt.a, stream));
       }
-      launch_sfxt_init(a, stream);
+      if (dataflow) launch_sfxt_dataflow_init(a, stream);
+      else launch_sfxt_init(a, stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.a, stream));
-      timing.sfxt_blocks = launch_sfxt_levels(a, level_loop_blocks(), stream);
+      timing.sfxt_blocks = dataflow ? launch_sfxt_dataflow(a, df_grid, stream)
+                                    : launch_sfxt_levels(a, level_loop_blocks(), stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.b, stream));
       d2h(&c, sctl.p, 1, stream);
       CPG_HIP(hipStreamSynchronize(stream));
@@ -495,11 +531,18 @@ struct cpg_handle {
     if (!loaded) throw std::runtime_error("no graph loaded");
     have_result = false;
     host_result = false;
+    host_tree = false;
     nres = 0;
+    nodes_stored = 0;
     timing = cpg_timing{};
     kernel_timing = o.kernel_timing != 0;
+    const bool step_times = o.step_times != 0;
+    step_ms.clear();
+    step_paths.clear();
     if (o.sfxt_blocks < 0) throw std::runtime_error("sfxt_blocks must be >= 0");
+    if (o.sfxt_method < 0 || o.sfxt_method > 1) throw std::runtime_error("sfxt_method must be 0 or 1");
     sfxt_blocks = o.sfxt_blocks;
+    sfxt_method = o.sfxt_method;
     const double tq0 = now_ms();
     timing.sfxt_ms = run_sfxt(true, o.endpoint_mask);
     if (k <= 0 || g.n == 0) {
@@ -568,6 +611,11 @@ struct cpg_handle {
       bool bound_from_store = probed;
       bool overflow = false;
       int r = 0;
+      if (step_times) {
+        rlog.reserve(kRoundLog + 1);
+        launch_round_log(pctl.p, rlog.p, kRoundLog + 1, stream);
+        CPG_HIP(hipEventRecord(round_event(0), stream));
+      }
       while (true) {
         if (r > 100000) throw std::runtime_error("PFXT round limit exceeded");
         if (now_ms() - tq0 > CPG_QUERY_LIMIT_MS) throw std::runtime_error("PFXT query exceeded its time limit");
@@ -608,6 +656,10 @@ struct cpg_handle {
             hist_nodes(1, false, true);
             launch_select_bound(pctl.p, hist_all.p, static_cast<unsigned long long>(k), stream);
           }
+          if (step_times && r < kRoundLog) {
+            launch_round_log(pctl.p, rlog.p, kRoundLog + 1, stream);
+            CPG_HIP(hipEventRecord(round_event(static_cast<size_t>(r) + 1), stream));
+          }
         }
         d2h(&c, pctl.p, 1, stream);
         CPG_HIP(hipStreamSynchronize(stream));
@@ -636,6 +688,18 @@ struct cpg_handle {
       timing.retries++;
     }
     timing.rounds = c.rounds;
+    if (step_times) {
+      const int nr = std::min(c.rounds, kRoundLog);
+      std::vector<unsigned long long> hl(static_cast<size_t>(nr) + 1);
+      d2h(hl.data(), rlog.p, static_cast<size_t>(nr) + 1, stream);
+      CPG_HIP(hipStreamSynchronize(stream));
+      for (int i = 0; i < nr; ++i) {
+        float ms = 0.0f;
+        CPG_HIP(hipEventElapsedTime(&ms, rev[static_cast<size_t>(i)], rev[static_cast<size_t>(i) + 1]));
+        step_ms.push_back(ms);
+        step_paths.push_back(static_cast<int64_t>(hl[i + 1] > hl[i] ? hl[i + 1] - hl[i] : 0));
+      }
+    }
     timing.expanded = static_cast<int64_t>(c.expanded);
     timing.generated = static_cast<int64_t>(c.node_n);
     timing.hops = static_cast<int64_t>(c.hops);
@@ -668,7 +732,12 @@ struct cpg_handle {
     launch_emit_report(sval[res].p, nout, ns, r_cost.p, r_level.p, r_from.p, r_to.p, r_pidx.p,
                        rank_of.p, stream);
     launch_link_parents(r_pidx.p, rank_of.p, nout, r_parent.p, stream);
+    r_node.reserve(std::max<uint64_t>(1, nout));
+    if (nout)
+      CPG_HIP(hipMemcpyAsync(r_node.p, sval[res].p, sizeof(uint32_t) * nout, hipMemcpyDeviceToDevice,
+                             stream));
     CPG_HIP(hipStreamSynchronize(stream));
+    nodes_stored = nn;
     const double te = now_ms();
     timing.select_ms = te - ts0;
     timing.query_ms = te - tq0;
@@ -690,6 +759,55 @@ struct cpg_handle {
       CPG_HIP(hipStreamSynchronize(stream));
     }
     host_result = true;
+  }
+
+  // The prefix tree on the host: the report's node slots and every stored
+  // node's deviation and parent (a path's ancestors may be unreported, e.g.
+  // zero-cost paths, so the walk follows node slots, not report ranks).
+  void fetch_tree() {
+    if (host_tree) return;
+    fetch_result();
+    const size_t nr = static_cast<size_t>(nres), nn = static_cast<size_t>(nodes_stored);
+    h_node.resize(nr);
+    hn_from.resize(nn); hn_to.resize(nn); hn_parent.resize(nn);
+    d2h(h_node.data(), r_node.p, nr, stream);
+    d2h(hn_from.data(), n_from.p, nn, stream);
+    d2h(hn_to.data(), n_to.p, nn, stream);
+    d2h(hn_parent.data(), n_parent.p, nn, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    host_tree = true;
+  }
+
+  // Vertices of reported path i: its root's suffix chain with every
+  // deviation (from -> to) spliced in at its from-vertex.
+  std::vector<int> path_vertices(int64_t i) {
+    fetch_tree();
+    if (i < 0 || i >= nres) throw std::runtime_error("path index out of range");
+    fetch_succ();
+    std::vector<std::pair<int, int>> devs;  // (from, to), leaf to root
+    int64_t x = h_node[static_cast<size_t>(i)];
+    for (int guard = 0;; ++guard) {
+      if (x < 0 || x >= static_cast<int64_t>(nodes_stored) || guard > g.n + 1)
+        throw std::runtime_error("prefix-tree parent chain is broken");
+      if (hn_from[x] < 0) break;
+      devs.emplace_back(hn_from[x], hn_to[x]);
+      x = hn_parent[x];
+    }
+    std::vector<int> path;
+    int v = hn_to[x];
+    size_t di = devs.size();
+    while (v != -1) {
+      path.push_back(v);
+      if (di > 0 && devs[di - 1].first == v) {
+        v = devs[di - 1].second;
+        --di;
+      } else {
+        v = h_succ[v];
+      }
+      if (path.size() > static_cast<size_t>(g.n) + 1) throw std::runtime_error("path does not end");
+    }
+    if (di != 0) throw std::runtime_error("deviation vertex not on the suffix chain");
+    return path;
   }
 
   // A_dev index: one entry per (64-source interval, destination row) with
@@ -888,7 +1006,10 @@ struct cpg_batch {
     timing = cpg_timing{};
     const double t0 = now_ms();
     uh.kernel_timing = false;
+    if (o.sfxt_blocks < 0) throw std::runtime_error("sfxt_blocks must be >= 0");
+    if (o.sfxt_method < 0 || o.sfxt_method > 1) throw std::runtime_error("sfxt_method must be 0 or 1");
     uh.sfxt_blocks = o.sfxt_blocks;
+    uh.sfxt_method = o.sfxt_method;
     uh.reset();
     timing.sfxt_ms = uh.run_sfxt(false);
     hipStream_t stream = uh.stream;
@@ -1890,6 +2011,25 @@ int cpg_load_text(cpg_handle* h, const char* path, int ignore_wgts) {
   });
 }
 
+int cpg_load_bin(cpg_handle* h, const char* path) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    EdgeList el;
+    read_bin(path ? path : "", el);
+    h->set_graph(el);
+    return 0;
+  });
+}
+
+int cpg_write_bin(cpg_handle* h, const char* path) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (!h->loaded) throw std::runtime_error("no graph loaded");
+    write_bin(h->g, path ? path : "");
+    return 0;
+  });
+}
+
 int cpg_load_edges(cpg_handle* h, int n, int64_t m, const int* eu, const int* ev, const float* ew) {
   CPG_CHECK_HANDLE(h, -1)
   return guarded<int>(-1, [&]() {
@@ -1996,36 +2136,7 @@ int64_t cpg_get_nodes(cpg_handle* h, int64_t k, cpg_node* out) {
 int64_t cpg_get_path_vertices(cpg_handle* h, int64_t i, int* out, int64_t cap) {
   CPG_CHECK_HANDLE(h, -1)
   return guarded<int64_t>(-1, [&]() {
-    h->fetch_result();
-    if (i < 0 || i >= h->nres) throw std::runtime_error("path index out of range");
-    h->fetch_succ();
-    // deviations from the leaf up to the root
-    std::vector<std::pair<int, int>> devs;  // (from, to)
-    int64_t r = i;
-    int source = -1;
-    for (int guard = 0; r >= 0; ++guard) {
-      if (guard > (1 << 26)) throw std::runtime_error("parent chain does not terminate");
-      if (h->h_from[r] < 0) { source = h->h_to[r]; break; }
-      devs.emplace_back(h->h_from[r], h->h_to[r]);
-      const int p = h->h_parent[r];
-      if (p < 0) throw std::runtime_error("parent path not among the reported paths");
-      r = p;
-    }
-    std::reverse(devs.begin(), devs.end());
-    std::vector<int> path;
-    int v = source;
-    size_t di = 0;
-    while (v != -1) {
-      path.push_back(v);
-      if (di < devs.size() && devs[di].first == v) {
-        v = devs[di].second;
-        ++di;
-      } else {
-        v = h->h_succ[v];
-      }
-      if (path.size() > static_cast<size_t>(h->g.n) + 1) throw std::runtime_error("path does not end");
-    }
-    if (di != devs.size()) throw std::runtime_error("deviation vertex not on the suffix chain");
+    const std::vector<int> path = h->path_vertices(i);
     const int64_t len = static_cast<int64_t>(path.size());
     if (len > cap) return -len - 1000;
     if (out) std::memcpy(out, path.data(), sizeof(int) * len);
@@ -2037,6 +2148,17 @@ int cpg_timings(cpg_handle* h, cpg_timing* out) {
   CPG_CHECK_HANDLE(h, -1)
   if (out) *out = h->timing;
   return 0;
+}
+
+int64_t cpg_step_times(cpg_handle* h, double* ms, int64_t* paths, int64_t cap) {
+  CPG_CHECK_HANDLE(h, -1)
+  const int64_t n = static_cast<int64_t>(h->step_ms.size());
+  const int64_t c = std::min(std::max<int64_t>(cap, 0), n);
+  for (int64_t i = 0; i < c; ++i) {
+    if (ms) ms[i] = h->step_ms[static_cast<size_t>(i)];
+    if (paths) paths[i] = h->step_paths[static_cast<size_t>(i)];
+  }
+  return n;
 }
 
 int cpg_sfxt(cpg_handle* h) {

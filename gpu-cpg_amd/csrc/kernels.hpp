@@ -19,6 +19,8 @@ struct SfxtCtl {
   unsigned long long bar_word;    // grid barrier: generation << 32 | queue tail when it completed
   unsigned long long nbig;      // deviation lists one wave sorts (front of big[])
   unsigned long long nbig_l;    // deviation lists one block sorts (back of big[])
+  unsigned qhead;               // dataflow loop: queue slots claimed
+  int pad;
 };
 
 struct SfxtArgs {
@@ -46,6 +48,9 @@ struct SfxtArgs {
   int* big;        // vertices whose deviation lists are sorted block-wide, n slots
   int2* lift;      // lift levels x n: {2^k-th successor, min delta over those 2^k}
   SfxtCtl* ctl;
+  unsigned long long* dfw;  // dataflow loop: one progress word per block (dfw_n slots)
+  int dfw_n;
+  int pad;
 };
 
 // Q endpoint subsets relaxed in one level loop (served queries): the
@@ -88,6 +93,11 @@ void launch_sfxt_levels_parts(const SfxtParts& parts, int total_blocks, hipStrea
 // Per-vertex init, then the cooperative level loop (returns the grid used).
 void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);
+// The barrier-free dataflow loop (same results as the level loop); args.dfw
+// must hold at least sfxt_dataflow_grid(blocks) words.  Returns the grid used.
+int sfxt_dataflow_grid(int blocks);
+void launch_sfxt_dataflow_init(const SfxtArgs& args, hipStream_t stream);
+int launch_sfxt_dataflow(const SfxtArgs& args, int blocks, hipStream_t stream);
 // Deviation records, their sort, the lift tables and the nsv pointers.
 // m: the graph's edge count (chooses thread- or wave-per-vertex records).
 void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream);
@@ -165,6 +175,8 @@ void launch_make_batch(const NodeStore& nodes, PfxtCtl* ctl, uint32_t* batch, in
 void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s);
 void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k,
                          hipStream_t s);
+// rlog[rounds] = paths stored once `rounds` rounds have completed (rounds < cap).
+void launch_round_log(const PfxtCtl* ctl, unsigned long long* rlog, int cap, hipStream_t s);
 
 // ---------------- multi-query PFXT (batched designs) ----------------
 constexpr int kMqBins = 1024;                 // per-query histogram bins

@@ -533,6 +533,19 @@ void launch_walk(const WalkArgs& a, bool fill, int grid, hipStream_t s) {
   CPG_HIP(hipGetLastError());
 }
 
+namespace {
+// Stored paths when `rounds` rounds have completed.
+__global__ void round_log_kernel(const PfxtCtl* ctl, unsigned long long* rlog, int cap) {
+  const int r = ctl->rounds;
+  if (r >= 0 && r < cap) rlog[r] = ctl->node_n < ctl->node_cap ? ctl->node_n : ctl->node_cap;
+}
+}  // namespace
+
+void launch_round_log(const PfxtCtl* ctl, unsigned long long* rlog, int cap, hipStream_t s) {
+  hipLaunchKernelGGL(round_log_kernel, dim3(1), dim3(1), 0, s, ctl, rlog, cap);
+  CPG_HIP(hipGetLastError());
+}
+
 void launch_select_bound(PfxtCtl* ctl, unsigned* hist_all, unsigned long long k, hipStream_t s) {
   hipLaunchKernelGGL(select_bound_kernel, dim3(1), dim3(kSelBlock), 0, s, ctl, hist_all, k);
   CPG_HIP(hipGetLastError());

@@ -268,6 +268,324 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dataflow loop: the same levelize / relax / successor results as the level
+// loop, without a grid barrier per level.
+//
+// A vertex is ready once every fanout has settled (its deps counter reaches
+// zero).  The block that settles a vertex's last fanout takes it into its
+// LDS pool and settles it in its next iteration, so a chain of dependent
+// levels costs one block iteration per level instead of one grid barrier.
+// Settling is pull-style, as relax_bu_step (gpucpg/gpucpg.cu:1167-1187) and
+// update_successors_bu (:2606-2628) read the fanouts: dist[v] = min over
+// reachable fanouts w of dist[w] + (int)(w_e * 10000.0f), succ[v] = the
+// largest such w at the minimum, level[v] = 1 + max level[w] (the peel depth
+// of bfs_td_step_privatized, :2215-2283).
+//
+// Everything one block reads of another's work is a write-once 64-bit state
+// word per vertex, state[w] = level << 32 | dist (all ones until settled),
+// read and written with device-scope atomic loads and stores; a reader that
+// still sees all ones (the store has not landed yet) polls again.  So no
+// fence orders anything: each value carries its own validity, and the deps
+// counters are device-scope atomics.
+//
+// Work distribution: per block an LDS pool of ready vertices with their CSR
+// ranges, loaded when they are released (the releasing thread loads the
+// ranges of every fanin it decrements, in the same round trip as the
+// atomics).  A pool larger than kDfKeep after an iteration spills the rest
+// to a global queue (slots written once, -1 until then); the sinks start
+// there.  A block with an empty pool claims a window of kDfBlock queue
+// slots.  Termination: each block publishes one word (kept releases << 32 |
+// vertices settled); a block that finds no work reads every block's word
+// twice (settled counts, then released counts and the queue tail) and
+// leaves when the settled total equals the released total (the sinks and
+// spilled vertices count in the queue tail).  Releases are published before
+// the released vertices can be settled anywhere, so the check cannot pass
+// while work is outstanding.  Vertices on a cycle are never released and
+// keep their initial values, as the level loop leaves them.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kDfBlock = 256;
+constexpr int kDfPool = 2048;   // LDS pool capacity (vertices)
+constexpr int kDfKeep = 512;    // pool entries kept after an iteration
+constexpr int kDfUnroll = 8;    // edge records (and their loads / atomics) in flight per step
+constexpr unsigned long long kStateEmpty = ~0ull;
+
+__device__ inline unsigned long long ld_dev64(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_dev32(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev32(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned ld_devu32(unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int wave_max_int(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline unsigned long long wave_sum64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) sfxt_dataflow_init_kernel(SfxtArgs a) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gthreads = gridDim.x * blockDim.x;
+  for (int v = gtid; v < a.n; v += gthreads) {
+    a.deps[v] = a.oadjp[v + 1] - a.oadjp[v];
+    a.level[v] = -1;
+    a.dist[v] = kIntMax;
+    a.succ[v] = -1;
+    a.distf[v] = static_cast<float>(kIntMax) / static_cast<float>(kScaleUp);
+    a.best[v] = kStateEmpty;
+    a.queue[v] = v < a.nsinks ? a.sinks[v] : -1;
+  }
+  for (int b = gtid; b < a.dfw_n; b += gthreads) a.dfw[b] = 0ull;
+  if (gtid == 0) a.ctl->qtail = static_cast<unsigned>(a.nsinks);
+}
+
+__global__ void __launch_bounds__(kDfBlock) sfxt_dataflow_kernel(SfxtArgs a) {
+  __shared__ int s_v[kDfPool];
+  __shared__ int4 s_r[kDfPool];   // {fanout begin, fanout end, fanin begin, fanin end}
+  __shared__ int s_pn;            // pool entries (a push past kDfPool spills at once)
+  __shared__ int s_pbase;         // entries left in the pool after this iteration's take
+  __shared__ int s_take;
+  __shared__ int s_win;           // a window of queue slots is held
+  __shared__ unsigned s_clo;      // window: slots [s_clo, s_clo + kDfBlock)
+  __shared__ unsigned s_tail;     // queue tail known to the block
+  __shared__ unsigned long long s_wd[kDfBlock / 64];  // window positions taken
+  __shared__ int s_proc;          // vertices settled this iteration
+  __shared__ int s_spill;         // pool entries spilled this iteration
+  __shared__ unsigned s_qbase;
+  __shared__ int s_exit;
+  __shared__ int s_check;         // this idle iteration checks for quiescence
+  __shared__ unsigned long long s_red[kDfBlock / 64];
+  static_assert(kDfBlock == 256, "window done bits are four 64-bit words");
+
+  SfxtCtl* ctl = a.ctl;
+  const int t = threadIdx.x;
+  const int lane = __lane_id();
+  const int wid = t >> 6;
+  const unsigned n = static_cast<unsigned>(a.n);
+  unsigned long long* state = a.best;
+  unsigned my_r = 0, my_p = 0;  // thread 0: kept releases, vertices settled
+  unsigned idle = 0;
+  int maxlev = -1;
+  float local_max = 0.0f;
+  if (t == 0) {
+    s_pn = 0;
+    s_win = 0;
+    s_clo = 0;
+    s_tail = 0;
+    s_exit = 0;
+  }
+  __syncthreads();
+
+  auto push = [&](int z, int4 rz) {
+    const int p = atomicAdd(&s_pn, 1);
+    if (p < kDfPool) {
+      s_v[p] = z;
+      s_r[p] = rz;
+    } else {
+      // pool full: straight to the global queue (counted in the tail first)
+      const unsigned q = atomicAdd(&ctl->qtail, 1u);
+      st_dev32(a.queue + q, z);
+    }
+  };
+
+  for (;;) {
+    // -- take: the pool's top entries, then window slots for idle threads
+    if (t == 0) {
+      s_proc = 0;
+      if (!s_win && s_pn == 0) {
+        const unsigned tail = ld_devu32(&ctl->qtail);
+        const unsigned head = ld_devu32(&ctl->qhead);
+        if (tail > head) {
+          s_clo = atomicAdd(&ctl->qhead, static_cast<unsigned>(kDfBlock));
+          s_win = 1;
+          s_tail = tail;
+          for (int i = 0; i < kDfBlock / 64; ++i) s_wd[i] = 0ull;
+        }
+      } else if (s_win && s_clo + kDfBlock > s_tail) {
+        s_tail = ld_devu32(&ctl->qtail);
+      }
+      const int pn = s_pn;
+      s_take = min(pn, kDfBlock);
+      s_pbase = pn - s_take;
+    }
+    __syncthreads();
+    int v = -1;
+    int4 r = make_int4(0, 0, 0, 0);
+    bool have_r = false;
+    bool taken = false;
+    if (t < s_take) {
+      v = s_v[s_pbase + t];
+      r = s_r[s_pbase + t];
+      have_r = true;
+    } else if (s_win && !((s_wd[wid] >> lane) & 1ull)) {
+      const unsigned slot = s_clo + static_cast<unsigned>(t);
+      if (slot >= n) {
+        taken = true;  // never written
+      } else if (slot < s_tail) {
+        const int x = ld_dev32(a.queue + slot);
+        if (x >= 0) {
+          v = x;
+          taken = true;
+        }
+      }
+    }
+    const unsigned long long tb = __ballot(taken);
+    if (lane == 0 && tb) s_wd[wid] |= tb;
+    __syncthreads();
+    if (t == 0) s_pn = s_pbase;
+    __syncthreads();
+
+    // -- settle v
+    if (v >= 0) {
+      if (!have_r) r = make_int4(a.oadjp[v], a.oadjp[v + 1], a.iadjp[v], a.iadjp[v + 1]);
+      int bd = kIntMax, bw = -1, lev = 0;
+      if (r.x == r.y && !(a.sink_mask && !a.sink_mask[v])) bd = 0;  // a chosen sink
+      for (int j = r.x; j < r.y; j += kDfUnroll) {
+        int w[kDfUnroll];
+        float wf[kDfUnroll];
+        unsigned long long st[kDfUnroll];
+#pragma unroll
+        for (int q = 0; q < kDfUnroll; ++q) {
+          const int e = min(j + q, r.y - 1);
+          w[q] = a.oadj[e];
+          wf[q] = a.owgt[e];
+        }
+#pragma unroll
+        for (int q = 0; q < kDfUnroll; ++q) st[q] = ld_dev64(state + w[q]);
+#pragma unroll
+        for (int q = 0; q < kDfUnroll; ++q) {
+          if (j + q >= r.y) continue;
+          unsigned spins = 0;
+          while (st[q] == kStateEmpty) {  // settled, store not landed yet
+            if (++spins > kSpinLimit) {
+              atomicExch(&ctl->err, 1);
+              st[q] = static_cast<uint32_t>(kIntMax);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            st[q] = ld_dev64(state + w[q]);
+          }
+          const int d = static_cast<int>(static_cast<uint32_t>(st[q]));
+          lev = max(lev, static_cast<int>(st[q] >> 32) + 1);
+          if (d != kIntMax) {
+            const int wi = static_cast<int>(wf[q] * static_cast<float>(kScaleUp));
+            const int c = static_cast<int>(static_cast<unsigned>(d) + static_cast<unsigned>(wi));
+            if (c < bd || (c == bd && w[q] > bw)) {
+              bd = c;
+              bw = w[q];
+            }
+          }
+        }
+      }
+      const float df = static_cast<float>(bd) / static_cast<float>(kScaleUp);
+      a.dist[v] = bd;
+      a.succ[v] = bw;
+      a.level[v] = lev;
+      a.distf[v] = df;
+      if (bd != kIntMax) local_max = fmaxf(local_max, fabsf(df));
+      maxlev = max(maxlev, lev);
+      st_dev64(state + v, (static_cast<unsigned long long>(static_cast<uint32_t>(lev)) << 32) |
+                              static_cast<uint32_t>(bd));
+      // release the fanins; load each one's CSR ranges with its atomic
+      for (int j = r.z; j < r.w; j += kDfUnroll) {
+        int z[kDfUnroll];
+#pragma unroll
+        for (int q = 0; q < kDfUnroll; ++q) z[q] = a.iedge[min(j + q, r.w - 1)].x;
+        int old[kDfUnroll];
+        int4 rz[kDfUnroll];
+#pragma unroll
+        for (int q = 0; q < kDfUnroll; ++q) {
+          old[q] = 0;
+          if (j + q < r.w) {
+            old[q] = atomicSub(&a.deps[z[q]], 1);
+            rz[q] = make_int4(a.oadjp[z[q]], a.oadjp[z[q] + 1], a.iadjp[z[q]], a.iadjp[z[q] + 1]);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kDfUnroll; ++q)
+          if (j + q < r.w && old[q] == 1) push(z[q], rz[q]);
+      }
+    }
+    const unsigned long long pb = __ballot(v >= 0);
+    if (lane == 0 && pb) atomicAdd(&s_proc, __popcll(pb));
+    __syncthreads();
+
+    // -- publish, spill the pool beyond kDfKeep, retire a finished window
+    if (t == 0) {
+      const int pool_now = min(s_pn, kDfPool);
+      const int keep = min(pool_now, kDfKeep);  // >= s_pbase (<= kDfKeep)
+      const int spill = pool_now - keep;
+      s_spill = spill;
+      if (spill > 0) s_qbase = atomicAdd(&ctl->qtail, static_cast<unsigned>(spill));
+      s_pn = keep;
+      my_r += static_cast<unsigned>(keep - s_pbase);
+      my_p += static_cast<unsigned>(s_proc);
+      st_dev64(a.dfw + blockIdx.x, (static_cast<unsigned long long>(my_r) << 32) | my_p);
+      if (s_win && (s_wd[0] & s_wd[1] & s_wd[2] & s_wd[3]) == ~0ull) s_win = 0;
+      if (s_proc == 0 && keep == 0) {
+        ++idle;
+      } else {
+        idle = 0;
+      }
+      s_exit = 0;
+      s_check = (idle & 3u) == 3u;
+      if (idle > kSpinLimit) {
+        atomicExch(&ctl->err, 1);
+        s_exit = 1;
+      } else if (idle > 0 && __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        s_exit = 1;
+      }
+    }
+    __syncthreads();
+    for (int i = t; i < s_spill; i += kDfBlock) st_dev32(a.queue + s_qbase + i, s_v[kDfKeep + i]);
+    if (s_exit) break;
+    if (s_proc == 0 && s_pn == 0) {
+      // no work here: check for quiescence every fourth idle iteration
+      // (thread 0 keeps the idle count; s_check is set before the barrier
+      // that the spill loop above is followed by)
+      __builtin_amdgcn_s_sleep(8);
+      if (s_check) {
+        unsigned long long sp = 0;
+        for (int b = t; b < a.dfw_n; b += kDfBlock) sp += ld_dev64(a.dfw + b) & 0xffffffffull;
+        sp = wave_sum64(sp);
+        if (lane == 0) s_red[wid] = sp;
+        __syncthreads();
+        const unsigned long long settled = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+        __syncthreads();
+        unsigned long long sr = 0;
+        for (int b = t; b < a.dfw_n; b += kDfBlock) sr += ld_dev64(a.dfw + b) >> 32;
+        if (t == 0) sr += ld_devu32(&ctl->qtail);
+        sr = wave_sum64(sr);
+        if (lane == 0) s_red[wid] = sr;
+        __syncthreads();
+        const unsigned long long released = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+        if (settled == released) break;
+        __syncthreads();
+      }
+    }
+  }
+  const int ml = wave_max_int(maxlev);
+  const float mx = wave_max(local_max);
+  if (lane == 0) {
+    if (ml >= 0) atomicMax(&ctl->num_levels, ml + 1);
+    atomicMax(&ctl->max_abs_distf_bits, __float_as_uint(mx));
+  }
+  if (t == 0) atomicAdd(&ctl->levelized, my_p);
+}
+
 // Per-vertex state before the multi-subset level loop.
 __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMulti m) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -723,6 +1041,36 @@ int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
   SfxtArgs a = args;
   void* params[] = {&a};
   CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_kernel, dim3(grid), dim3(kSfxtBlock), params, 0,
+                                     stream));
+  return grid;
+}
+
+int sfxt_dataflow_grid(int blocks) {
+  int per_cu = 0;
+  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_dataflow_kernel, kDfBlock, 0));
+  if (per_cu < 1) throw std::runtime_error("sfxt dataflow kernel cannot be resident");
+  const int cus = device_cus();
+  const int want = blocks > 0 ? blocks : cus;
+  return std::max(1, std::min(want, cus * per_cu));
+}
+
+void launch_sfxt_dataflow_init(const SfxtArgs& args, hipStream_t stream) {
+  const int items = std::max(args.n, args.dfw_n);
+  hipLaunchKernelGGL(sfxt_dataflow_init_kernel, dim3(flat_grid(items, device_cus())), dim3(256), 0,
+                     stream, args);
+  CPG_HIP(hipGetLastError());
+}
+
+int launch_sfxt_dataflow(const SfxtArgs& args, int blocks, hipStream_t stream) {
+  int dev = 0, coop = 0;
+  CPG_HIP(hipGetDevice(&dev));
+  CPG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) throw std::runtime_error("device does not support cooperative launch");
+  const int grid = sfxt_dataflow_grid(blocks);
+  if (args.dfw_n < grid || !args.dfw) throw std::runtime_error("dataflow progress words too few");
+  SfxtArgs a = args;
+  void* params[] = {&a};
+  CPG_HIP(hipLaunchCooperativeKernel(sfxt_dataflow_kernel, dim3(grid), dim3(kDfBlock), params, 0,
                                      stream));
   return grid;
 }

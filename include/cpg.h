@@ -56,6 +56,11 @@ typedef struct {
   int bound_probe;          /* first bound from pruned count passes: 0 = when the
                                roots' chains are long (automatic), 1 = always,
                                -1 = never (the first round walks unbounded)   */
+  int sfxt_method;          /* 0: barrier-free dataflow loop (default), 1: the
+                               level-synchronous loop (one grid barrier per
+                               level); both give identical results            */
+  int step_times;           /* 1: record every expansion round's device time and
+                               the paths it stored (cpg_step_times)           */
 } cpg_query_opts;
 
 /* Timings of the last query, milliseconds (gpucpg/gpucpg.cuh:327-356). */
@@ -91,6 +96,13 @@ const char* cpg_last_error(void);                  /* thread-local message */
 /* ---- graph input ---- */
 /* CpGen::read_input, gpucpg/gpucpg.cu:453-548. */
 int cpg_load_text(cpg_handle* h, const char* path, int ignore_wgts);
+/* Binary fanout CSR (gpucpg/graph.h:24-55: int32 N, int32 M, offsets[N+1],
+ * adjncy[M], float wgts[M], native-endian), the format
+ * CpGen::write_to_csr_bin (gpucpg/gpucpg.cuh:310-323) writes and
+ * Graph::read_from_bin reads; the fanin CSR is derived as for text input. */
+int cpg_load_bin(cpg_handle* h, const char* path);
+/* Writes the loaded graph's fanout CSR in that format. */
+int cpg_write_bin(cpg_handle* h, const char* path);
 /* Edge list in input order: same CSR semantics as the text loader. */
 int cpg_load_edges(cpg_handle* h, int n, int64_t m, const int* eu,
                    const int* ev, const float* ew);
@@ -122,6 +134,12 @@ int64_t cpg_get_nodes(cpg_handle* h, int64_t k, cpg_node* out);
  * counterpart in the reference (its parents are stale after the sort). */
 int64_t cpg_get_path_vertices(cpg_handle* h, int64_t i, int* out, int64_t cap);
 int cpg_timings(cpg_handle* h, cpg_timing* out);
+/* Per-round record of the last query run with step_times: ms[i] the device
+ * time of expansion round i (HIP events on the engine's stream), paths[i]
+ * the prefix-tree paths it stored (NULL arrays are skipped).  Returns the
+ * number of rounds.  CpGen::short_long_step_times and paths_gen_per_step,
+ * gpucpg/gpucpg.cuh:351-352 (filled at gpucpg/gpucpg.cu:20404-20405). */
+int64_t cpg_step_times(cpg_handle* h, double* ms, int64_t* paths, int64_t cap);
 
 /* ---- SFXT results (dist/succ/level of the last SFXT) ---- */
 /* dist: int-scaled suffix distance (relax_bu_step, gpucpg.cu:1167-1187),

@@ -329,3 +329,69 @@ def test_medium_fanout_deviation_sorts():
         g.reset()
         g.report_paths(k)
         assert_bits_equal(want, g.get_slacks(k), f"medium fanout K={k}")
+
+
+def star_graph(fan=5000, seed=5):
+    """A sink with `fan` fanins, each fed by a source and by one hub:
+    settling the sink releases `fan` vertices at once (more than one block's
+    LDS pool holds), so the dataflow loop spills to its global queue."""
+    rng = np.random.default_rng(seed)
+    n = 2 + 2 * fan
+    u, v = [], []
+    for i in range(fan):
+        u += [2 + i, 2 + fan + i, 1]
+        v += [0, 2 + i, 2 + i]
+    w = rng.uniform(0.01, 1.0, len(u)).astype(np.float32)
+    return n, np.array(u, np.int32), np.array(v, np.int32), w
+
+
+@pytest.mark.parametrize("blocks", [1, 2, 0])
+def test_sfxt_dataflow_pool_overflow(blocks):
+    n, u, v, w = star_graph()
+    say(f"case star graph dataflow blocks={blocks}")
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, level, L = og.sfxt()
+    g = engine_for(n, u, v, w)
+    g.report_paths(1000, sfxt_blocks=blocks)
+    gd, gs, gl = g.sfxt()
+    np.testing.assert_array_equal(gd, dist)
+    np.testing.assert_array_equal(gs, succ)
+    np.testing.assert_array_equal(gl, level)
+    assert g.graph_diameter == L
+    want, _ = og.topk(dist, succ, 1000)
+    assert_bits_equal(want, g.get_slacks(1000), f"star blocks={blocks}")
+
+
+@pytest.mark.parametrize("which", [1, 2, 3])
+def test_sfxt_methods_agree(which):
+    """The dataflow loop (default) and the level-synchronous loop give the
+    oracle's dist/succ/level on C1, C2 and the densified C3."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(which)
+    g = engine_for(n, u, v, w)
+    for method in (0, 1):
+        say(f"case C{which} sfxt method {method}")
+        g.reset()
+        g.report_paths(10, sfxt_method=method, kernel_timing=True)
+        gd, gs, gl = g.sfxt()
+        np.testing.assert_array_equal(gd, dist)
+        np.testing.assert_array_equal(gs, succ)
+        np.testing.assert_array_equal(gl, level)
+        assert g.graph_diameter == L
+        say(f"C{which} method {method} levels_ms {g.timings()['sfxt_levels_ms']:.3f}")
+
+
+def test_sfxt_dataflow_cycle_terminates():
+    """A cycle never releases its vertices: the loop ends by quiescence and
+    leaves them unsettled, as the level loop does."""
+    u = np.array([0, 1, 2, 2, 4], np.int32)
+    v = np.array([1, 2, 1, 3, 3], np.int32)
+    w = np.array([1, 2, 3, 4, 5], np.float32)
+    say("case cycle graph")
+    res = []
+    for method in (0, 1):
+        g = engine_for(5, u, v, w)
+        g.report_paths(10, sfxt_method=method)
+        res.append(g.sfxt() + (g.get_slacks(10),))
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(res[0][2], [-1, -1, -1, 0, 1])
