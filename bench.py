@@ -108,6 +108,9 @@ def batched_section(np, cpgamd, reps=3):
                 if np.array_equal(g.get_slacks(d[4]).view(np.uint32),
                                   b.get_slacks(q, d[4]).view(np.uint32)))
     best = min(batch_ms[1:]) if len(batch_ms) > 1 else batch_ms[0]
+    b.close()
+    for g in hs:
+        g.close()
     return {"workload": "256 designs, 5K-200K vertices each (13.5M total), K 100-100K (3.73M total)",
             "batch_ms": round(best, 3), "queries_per_s": round(256 / (best * 1e-3), 1),
             "plain_loop_ms": round(plain_ms, 3),
@@ -149,6 +152,7 @@ def bvss_section(np, cpgamd, nfront=16, span=4096, reps=3):
             out[name].update({"tiles": tiles, "int8_ops": ops, "achieved_TOPS": round(tops, 3),
                               "peak_TOPS": 5000.0, "frac": round(tops / 5000.0, 6)})
     out["exact"] = all(np.array_equal(a, b) for a, b in zip(res["bitmap"], res["mfma"]))
+    g.close()
     return out
 
 
@@ -243,6 +247,10 @@ def served_section(np, cpgamd, nq=48, load=0.8):
         if busy == 0 and nxt < nq:  # idle until the next arrival
             time.sleep(max(0.0, min(0.002, (start + arrive[nxt] - now()) * 1e-3)))
     served_span = now() - start
+    server_stats = S.stats()
+    S.close()
+    for g in hs:
+        g.close()
     lp = np.array(lat_plain)
     ls = np.array(list(lat_served.values()))
     return {"workload": "8 graphs (200K-2.6M vertices, 4M-70M edges), %d queries, endpoint subsets "
@@ -258,7 +266,7 @@ def served_section(np, cpgamd, nq=48, load=0.8):
                         "plain": {"p50_ms": round(float(np.percentile(lp, 50)), 2),
                                   "p99_ms": round(float(np.percentile(lp, 99)), 2),
                                   "queries_per_s": round(nq / (plain_span * 1e-3), 2)}},
-            "exact": "%d/%d" % (exact, nq), "server": S.stats()}
+            "exact": "%d/%d" % (exact, nq), "server": server_stats}
 
 
 def main():
@@ -403,7 +411,7 @@ def main():
             "nproc": os.cpu_count()}
         out["exact"] = bool(len(cpu_costs) == len(costs)
                             and pyoracle.first_bit_mismatch(cpu_costs, costs) == -1)
-    del g
+    g.close()  # every engine object is closed before exit, none from a finalizer
     if not args.no_batched:
         out["batched"] = batched_section(np, cpgamd)
     if not args.no_served:

@@ -8,6 +8,7 @@ fails with an exception when the library has not been built.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -28,7 +29,8 @@ class QueryOpts(ctypes.Structure):
                 ("endpoint_mask", ctypes.c_void_p), ("collect_stats", ctypes.c_int),
                 ("kernel_timing", ctypes.c_int), ("sfxt_blocks", ctypes.c_int),
                 ("bound_probe", ctypes.c_int), ("sfxt_method", ctypes.c_int),
-                ("step_times", ctypes.c_int)]
+                ("step_times", ctypes.c_int), ("monotone_check", ctypes.c_int),
+                ("pad", ctypes.c_int)]
 
 
 class Timing(ctypes.Structure):
@@ -42,7 +44,8 @@ class Timing(ctypes.Structure):
                 ("walk_fill_ms", ctypes.c_double), ("walk_launches", ctypes.c_int64),
                 ("sfxt_levels_ms", ctypes.c_double), ("sfxt_blocks", ctypes.c_int64),
                 ("probes", ctypes.c_int64), ("bvss_ms", ctypes.c_double),
-                ("bvss_tiles", ctypes.c_int64)]
+                ("bvss_tiles", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
+                ("node_slots", ctypes.c_int64), ("nonmonotone", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -51,7 +54,8 @@ class Timing(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "submitted", "admitted", "completed", "rounds", "compactions", "restarts",
-        "peak_nodes", "node_cap", "peak_index_bytes", "in_flight", "queued", "sfxt_groups", "root_bounds")] + [
+        "peak_nodes", "node_cap", "peak_index_bytes", "in_flight", "queued", "sfxt_groups",
+        "nonmonotone", "root_bounds")] + [
         (n, ctypes.c_double) for n in ("sfxt_ms", "group_wait_ms", "probe_ms", "round_ms",
                                        "extract_ms", "compact_ms")]
 
@@ -214,11 +218,23 @@ class CpGen:
         self._k = 0
 
     def close(self):
+        """Frees the handle's device memory, stream and events now.  Objects
+        still open when the interpreter shuts down are left to the process
+        exit: no HIP call is made from a finalizer then (the HIP runtime or
+        a profiler attached to it may already be torn down)."""
         if self.h:
             self.L.cpg_destroy(self.h)
             self.h = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -273,7 +289,7 @@ class CpGen:
     def report_paths(self, k, *args, root_tie_quirk=True, batch=0, max_nodes=0,
                      collect_stats=False, init_split_perc=0.005, kernel_timing=False,
                      sfxt_blocks=0, bound_probe=0, endpoint_mask=None, sfxt_method=0,
-                     step_times=False, **kwargs):
+                     step_times=False, monotone_check=False, mem_budget_bytes=0, **kwargs):
         """Runs the GPG query for the k lowest-cost paths.  Positional
         arguments after k (max_dev_lvls, enable_compress, methods, ...) are
         accepted for call compatibility with the reference; the engine always
@@ -292,6 +308,8 @@ class CpGen:
         o.bound_probe = int(bound_probe)
         o.sfxt_method = int(sfxt_method)
         o.step_times = 1 if step_times else 0
+        o.monotone_check = 1 if monotone_check else 0
+        o.mem_budget_bytes = int(mem_budget_bytes)
         mask = None
         if endpoint_mask is not None:
             mask = np.ascontiguousarray(endpoint_mask, dtype=np.uint8)
@@ -420,11 +438,22 @@ class Batch:
         if not self.h:
             raise RuntimeError(self.L.cpg_last_error().decode())
 
+    def close(self):
+        if self.h:
+            self.L.cpg_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
-            if self.h:
-                self.L.cpg_batch_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 
@@ -466,11 +495,22 @@ class Server:
             raise CpgError(self.L.cpg_last_error().decode())
         self._n = []
 
+    def close(self):
+        if self.h:
+            self.L.cpg_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
-            if self.h:
-                self.L.cpg_server_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 

@@ -417,6 +417,27 @@ struct cpg_handle {
     return now_ms() - t0;
   }
 
+  // Device bytes per node-store slot: cost, to, from, parent, level, key (4
+  // each), flag (1), count-pass key and batch entry (4 each), and the final
+  // sort's two key/value buffer pairs (24) and rank (4).
+  static constexpr uint64_t kNodeBytes = 61;
+
+  // Device bytes the graph and the SFXT hold.
+  uint64_t sfxt_device_bytes() const {
+    auto b = [](const auto& d) { return static_cast<uint64_t>(d.n) * sizeof(*d.p); };
+    return b(oadjp) + b(oadj) + b(owgt) + b(iadjp) + b(iedge) + b(srcs) + b(sinks) + b(deps) +
+           b(queue) + b(lvlp) + b(level) + b(dist) + b(succ) + b(src_dist) + b(big) + b(best) +
+           b(distf) + b(vrec) + b(devs) + b(devs_tmp) + b(devinfo) + b(lift) + b(sctl) + b(dfw) +
+           b(emask) + b(keep);
+  }
+
+  void release_pfxt_buffers() {
+    n_cost.release(); n_to.release(); n_from.release(); n_parent.release(); n_level.release();
+    n_key.release(); n_flag.release(); ckeys.release(); batch.release();
+    skey[0].release(); skey[1].release(); sval[0].release(); sval[1].release(); rank_of.release();
+    node_cap = 0;
+  }
+
   void ensure_pfxt_buffers(uint64_t cap) {
     if (cap > 0x7fffffffull) throw std::runtime_error("node capacity exceeds 2^31");
     n_cost.reserve(cap); n_to.reserve(cap); n_from.reserve(cap);
@@ -573,6 +594,20 @@ struct cpg_handle {
                                                         16ull * static_cast<uint64_t>(k) +
                                                             static_cast<uint64_t>(nroots) + (1u << 20));
     cap = std::min<uint64_t>(cap, 0x7fffffffull);
+    // Device-byte budget: the graph and its SFXT are held already; the node
+    // store and the final sort take kNodeBytes per slot, so the store gets
+    // what is left, and a store the budget cannot grow ends the query.
+    uint64_t budget_cap = 0x7fffffffull;
+    if (o.mem_budget_bytes > 0) {
+      const uint64_t held = sfxt_device_bytes();
+      const uint64_t budget = static_cast<uint64_t>(o.mem_budget_bytes);
+      budget_cap = budget > held ? std::min<uint64_t>((budget - held) / kNodeBytes, 0x7fffffffull) : 0;
+      if (budget_cap < static_cast<uint64_t>(nroots) + 1024)
+        throw std::runtime_error("mem_budget_bytes leaves no room for the prefix-tree node store (graph "
+                                 "and SFXT hold " + std::to_string(held) + " bytes)");
+      cap = std::min(cap, budget_cap);
+      if (node_cap > cap) release_pfxt_buffers();
+    }
     PfxtCtl c{};
     const bool trace = std::getenv("CPG_TRACE") != nullptr;
     // Rounds with a finite bound walk once: the fill pass stores children at
@@ -682,12 +717,20 @@ struct cpg_handle {
         if (c.done || c.batch_n == 0) break;  // no unexpanded path below the bound
       }
       if (!overflow) break;
+      if (cap >= budget_cap)
+        throw std::runtime_error("prefix-tree node store would exceed mem_budget_bytes (" +
+                                 std::to_string(cap) + " slots fit)");
       if (attempt >= 4 || cap >= 0x7fffffffull) throw std::runtime_error("prefix-tree node capacity exceeded");
-      cap = std::min<uint64_t>(cap * 2, 0x7fffffffull);
+      cap = std::min<uint64_t>({cap * 2, 0x7fffffffull, budget_cap});
+      if (o.mem_budget_bytes > 0) release_pfxt_buffers();  // reallocated at the new size
       two_pass = true;
       timing.retries++;
     }
     timing.rounds = c.rounds;
+    timing.nonmonotone = static_cast<int64_t>(c.nonmono);
+    if (o.monotone_check && c.nonmono)
+      throw std::runtime_error("a child path cost less than its parent (" + std::to_string(c.nonmono) +
+                               " times): the top K may miss a path just below the bound");
     if (step_times) {
       const int nr = std::min(c.rounds, kRoundLog);
       std::vector<unsigned long long> hl(static_cast<size_t>(nr) + 1);
@@ -738,6 +781,9 @@ struct cpg_handle {
                              stream));
     CPG_HIP(hipStreamSynchronize(stream));
     nodes_stored = nn;
+    timing.device_bytes = static_cast<int64_t>(
+        sfxt_device_bytes() + node_cap * kNodeBytes);
+    timing.node_slots = static_cast<int64_t>(node_cap);
     const double te = now_ms();
     timing.select_ms = te - ts0;
     timing.query_ms = te - tq0;
@@ -1125,6 +1171,10 @@ struct cpg_batch {
       timing.retries++;
     }
     timing.generated = static_cast<int64_t>(mc.node_n);
+    timing.nonmonotone = static_cast<int64_t>(mc.nonmono);
+    if (o.monotone_check && mc.nonmono)
+      throw std::runtime_error("a child path cost less than its parent (" + std::to_string(mc.nonmono) +
+                               " times): a query's top K may miss a path just below its bound");
     const double ts0 = now_ms();
     timing.pfxt_ms = ts0 - tp0;
     // final selection: (query, cost, level) sort of the paths at or below
@@ -1285,6 +1335,7 @@ struct cpg_server {
   DevBuf<unsigned long long> nsel, per_q;
   DevBuf<float> out_cost;
   cpg_server_stats stats{};
+  int64_t nonmono_base = 0;       // non-monotone children counted before the last ctl reset
   bool ready = false;
   hipStream_t sstream = nullptr;  // level-loop launches of admission groups
   hipEvent_t ev_levels = nullptr;
@@ -1650,8 +1701,17 @@ struct cpg_server {
     if (gis.empty()) return;
     std::vector<std::vector<unsigned>> maxd(gis.size());
     std::vector<std::vector<Query*>> qp(gis.size());  // looked up here, not in the threads
+    // every group is checked before any is committed: a failed check puts
+    // the queries of all these groups back on the queue with their slots
+    // and index budget released, then reports the error
+    try {
+      for (size_t i = 0; i < gis.size(); ++i) maxd[i] = check_group(gis[i]);
+    } catch (...) {
+      for (auto it = gis.rbegin(); it != gis.rend(); ++it) unwind_group(*it);
+      throw;
+    }
     for (size_t i = 0; i < gis.size(); ++i) {
-      maxd[i] = check_group(gis[i]);
+      graphs[gis[i]]->busy = false;
       for (int64_t id : graphs[gis[i]]->ids) qp[i].push_back(queries.at(id).get());
     }
     auto run = [&](size_t i, std::mutex* mu) {
@@ -1710,7 +1770,6 @@ struct cpg_server {
       d2h(maxd.data(), G.mm.maxd.p, Q, h.stream);
     }
     CPG_HIP(hipStreamSynchronize(h.stream));
-    G.busy = false;
     if (h.g.n > 0) {
       if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
       if (c.num_levels != h.diameter) throw std::runtime_error("SFXT levels differ from the graph's");
@@ -1721,6 +1780,27 @@ struct cpg_server {
     }
     stats.sfxt_ms += now_ms() - t0;
     return maxd;
+  }
+
+  // Returns a started group's queries to the front of the queue (in their
+  // order), with their reserved slots, index buffers and budget released.
+  void unwind_group(int gi) {
+    Graph& G = *graphs[gi];
+    for (auto it = G.ids.rbegin(); it != G.ids.rend(); ++it) {
+      Query& q = *queries.at(*it);
+      if (q.slot >= 0) {
+        slot_reserved[q.slot] = 0;
+        free_slots.push_back(q.slot);
+      }
+      q.slot = -1;
+      q.state = 0;
+      if (q.index) G.pool.push_back(std::move(q.index));
+      index_bytes -= G.index_bytes;
+      queue.push_front(q.id);
+    }
+    G.ids.clear();
+    G.args.clear();
+    G.busy = false;
   }
 
   // Copies the live nodes (slots not done) into the other buffer set, of
@@ -1745,6 +1825,7 @@ struct cpg_server {
     node_n = live;
     MqCtl c{};
     c.node_n = live;
+    nonmono_base = stats.nonmonotone;
     h2d(mctl.p, &c, 1, stream);
     batch.reserve(st[cur].cap);
     CPG_HIP(hipStreamSynchronize(stream));
@@ -1919,6 +2000,7 @@ struct cpg_server {
     batch.reserve(c);
     node_n = 0;
     const MqCtl z{};
+    nonmono_base = stats.nonmonotone;
     h2d(mctl.p, &z, 1, stream);
     CPG_HIP(hipStreamSynchronize(stream));
     stats.restarts++;
@@ -1940,6 +2022,7 @@ struct cpg_server {
       stats.rounds++;
       stats.round_ms += now_ms() - t0;
       if (mc.err) throw std::runtime_error("served PFXT walk exceeded its step bound");
+      stats.nonmonotone = nonmono_base + static_cast<int64_t>(mc.nonmono);
       node_n = mc.node_n;
       stats.peak_nodes = std::max<int64_t>(stats.peak_nodes, static_cast<int64_t>(std::min(node_n, st[cur].cap)));
       if (mc.overflow) {

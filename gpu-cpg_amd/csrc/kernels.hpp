@@ -123,6 +123,7 @@ struct PfxtCtl {
   int overflow;                    // node capacity exceeded in a fill pass
   int rounds;
   int err;                         // a walk exceeded its step bound (corrupt index)
+  unsigned long long nonmono;      // children whose cost came out below their parent's
 };
 
 // Prefix-tree node store: one slot per generated path, roots first.
@@ -209,6 +210,7 @@ struct MqCtl {
   int err;
   int active;                     // queries with unexpanded paths this round
   int pad;
+  unsigned long long nonmono;     // children whose cost came out below their parent's
 };
 
 struct MqArgs {

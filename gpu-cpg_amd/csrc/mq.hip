@@ -172,7 +172,7 @@ template <bool kFill>
 __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
   MqCtl* ctl = a.ctl;
   const uint64_t nb = ctl->batch_n;
-  unsigned long long hops = 0, devv = 0;
+  unsigned long long hops = 0, devv = 0, nonmono = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < nb;
        base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t i = base + threadIdx.x;
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
       float cc = 0.0f;
       int cw = 0, from = 0;
       const bool emit = w.step(qs.vrec, qs.devinfo, qs.devs, U, !kFill, qs.step_limit, &ctl->err,
-                               &hops, &devv, &cc, &cw, &from);
+                               &hops, &devv, &nonmono, &cc, &cw, &from);
       if (!kFill) {
         if (emit) {
           const uint32_t k = float_key(cc);
@@ -227,6 +227,8 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
   }
   (void)hops;
   (void)devv;
+  for (int o = 32; o > 0; o >>= 1) nonmono += __shfl_xor(nonmono, o);
+  if (kFill && __lane_id() == 0 && nonmono) atomicAdd(&ctl->nonmono, nonmono);  // once per expansion
 }
 
 // One block per query: U_q = the largest key of the bin where the running

@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
   const float U = key_float(ukey);
   const uint64_t cap = a.nodes.cap;
   const float mag = 2.0f * a.max_abs_distf + a.max_abs_wgt + 1.0f;
-  unsigned long long hops = 0, devv = 0;
+  unsigned long long hops = 0, devv = 0, nonmono = 0;
   int staged = 0;  // wave-uniform
 
   // Writes the wave's staged children: one atomic on node_n (fill) or
@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
       float cc = 0.0f;
       int cw = 0, from = 0;
       const bool emit = cw_state.step(a.vrec, a.devinfo, a.devs, U, a.strict != 0, a.step_limit,
-                                      &ctl->err, &hops, &devv, &cc, &cw, &from);
+                                      &ctl->err, &hops, &devv, &nonmono, &cc, &cw, &from);
       const uint64_t m = __ballot(emit);
       if (m) {
         const int add = static_cast<int>(__popcll(m));
@@ -385,6 +385,8 @@ __global__ void __launch_bounds__(kBlock) walk_kernel(WalkArgs a) {
     }
   }
   flush();
+  for (int o = 32; o > 0; o >>= 1) nonmono += __shfl_xor(nonmono, o);
+  if (kFill && lane == 0 && nonmono) atomicAdd(&ctl->nonmono, nonmono);  // once per expansion
   if (a.collect_stats) {
     for (int o = 32; o > 0; o >>= 1) {
       hops += __shfl_xor(hops, o);

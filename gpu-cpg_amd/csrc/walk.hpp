@@ -37,10 +37,15 @@ struct ChainWalk {
   // when !strict); *cc, *cw and *from receive its cost, its deviation head
   // w and the deviation vertex u.  A walk visits each chain vertex and each
   // deviation at most once, so more than step_limit steps means a corrupt
-  // index: the walk stops and *err is raised.
+  // index: the walk stops and *err is raised.  *nonmono counts the children
+  // whose binary32 cost comes out below the parent's (a deviation whose
+  // int-scaled delta is 0, or a negative weight): where one sits at the
+  // bound, a path just below U can descend from a parent at or above U,
+  // which is never expanded.
   __device__ inline bool step(const int4* vrec, const int2* devinfo, const int4* devs, float U,
                               bool strict, int step_limit, int* err, unsigned long long* hops,
-                              unsigned long long* devv, float* cc, int* cw, int* from) {
+                              unsigned long long* devv, unsigned long long* nonmono, float* cc,
+                              int* cw, int* from) {
     if (mode == 2) return false;
     if (++steps > step_limit) {
       atomicExch(err, 1);
@@ -74,6 +79,7 @@ struct ChainWalk {
       } else {
         const float x = ((c + __int_as_float(d.z)) + __int_as_float(d.y)) - dv;
         ++j;
+        if (x < c) ++*nonmono;
         if (strict ? x < U : x <= U) {
           emit = true;
           *cc = x;

@@ -47,7 +47,10 @@ typedef struct {
                                (int)(p*n+1) as gpucpg.cu:19141-19180 does     */
   int64_t batch;            /* parents expanded per round (0 = automatic)     */
   int64_t max_nodes;        /* prefix-tree node capacity (0 = automatic)      */
-  int64_t mem_budget_bytes; /* device bytes a query may hold (0 = no limit)   */
+  int64_t mem_budget_bytes; /* device bytes a query may hold (0 = no limit): the
+                               prefix-tree node store and the final sort get
+                               what the graph and its SFXT leave; a query whose
+                               store would have to grow past it fails          */
   const uint8_t* endpoint_mask; /* optional n-byte sink subset (NULL = all sinks) */
   int collect_stats;        /* 1: count hops / deviations / emitted children  */
   int kernel_timing;        /* 1: time the SFXT and walk kernels with HIP events
@@ -61,6 +64,10 @@ typedef struct {
                                level); both give identical results            */
   int step_times;           /* 1: record every expansion round's device time and
                                the paths it stored (cpg_step_times)           */
+  int monotone_check;       /* 1: fail the query (error) when a child's binary32
+                               cost came out below its parent's; 0: only count
+                               them (cpg_timing.nonmonotone)                  */
+  int pad;
 } cpg_query_opts;
 
 /* Timings of the last query, milliseconds (gpucpg/gpucpg.cuh:327-356). */
@@ -86,6 +93,13 @@ typedef struct {
   int64_t probes;         /* bound-probe fill passes run before the first round            */
   double bvss_ms;         /* last cpg_bvss_discover: device time (frontier build + both paths) */
   int64_t bvss_tiles;     /* last cpg_bvss_discover: 16x16x64 int8 MFMA tiles issued        */
+  int64_t device_bytes;   /* device memory the handle holds after the query (graph, SFXT,
+                             node store, final sort)                                          */
+  int64_t node_slots;     /* prefix-tree node store capacity (61 device bytes per slot with
+                             the final sort's buffers)                                        */
+  int64_t nonmonotone;    /* children whose binary32 cost came out below their parent's: the
+                             top K is the exact K cheapest whenever none of them sits at the
+                             bound; the reference's own best-first assumes there are none */
 } cpg_timing;
 
 /* ---- lifetime ---- */
@@ -208,6 +222,7 @@ typedef struct {
   int64_t peak_index_bytes;  /* in-flight deviation indexes, device bytes    */
   int64_t in_flight, queued;
   int64_t sfxt_groups;       /* level loops run (each for a group of queries)  */
+  int64_t nonmonotone;       /* children whose cost came out below their parent's */
   int64_t root_bounds;       /* queries whose first bound came from their roots */
   double sfxt_ms;            /* host time starting and checking groups' SFXTs  */
   double group_wait_ms;      /* host time waiting for a group, none in flight  */

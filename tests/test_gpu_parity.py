@@ -395,3 +395,77 @@ def test_sfxt_dataflow_cycle_terminates():
     for a, b in zip(res[0], res[1]):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(res[0][2], [-1, -1, -1, 0, 1])
+
+
+NM_GRID = [0.1, 0.2, 0.3, 0.7, 0.05, 0.15, 0.25, 0.35, 0.45, 0.6, 1.1, 1.3]
+
+
+def decimal_grid_dag(seed, n=10, m=22):
+    """Tiny DAG with weights on a decimal grid: int-scaled ties are common
+    and binary32 rounds some deviations below their parent's cost (e.g.
+    distf 0.1 + weight 0.2 < distf 0.3 in binary32)."""
+    rng = np.random.default_rng(seed)
+    edges = []
+    for _ in range(m):
+        a, b = sorted(rng.choice(n, 2, replace=False))
+        edges.append((int(a), int(b), float(np.float32(rng.choice(NM_GRID)))))
+    return (n, np.array([e[0] for e in edges], np.int32), np.array([e[1] for e in edges], np.int32),
+            np.array([e[2] for e in edges], np.float32))
+
+
+@pytest.mark.parametrize("seed", [5, 12, 20, 31, 57])
+def test_nonmonotone_children_counted_and_checked(seed):
+    """With K past the number of paths every path is expanded: the engine
+    finds every path (the brute-force list) and counts exactly the children
+    the oracle counts below their parent; monotone_check turns that into an
+    error."""
+    n, u, v, w = decimal_grid_dag(seed)
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, _, _ = og.sfxt()
+    allp = og.all_paths(dist, succ)
+    k = len(allp) + 5
+    want, st = og.topk(dist, succ, k, root_tie_quirk=False)
+    assert st["nonmonotone"] > 0
+    np.testing.assert_array_equal(want, allp)
+    say(f"case non-monotone seed {seed}: {st['nonmonotone']} children, {len(allp)} paths")
+    g = engine_for(n, u, v, w)
+    g.report_paths(k, root_tie_quirk=False)
+    assert_bits_equal(allp, g.get_slacks(k), f"non-monotone seed {seed}")
+    assert g.timings()["nonmonotone"] == st["nonmonotone"]
+    with pytest.raises(cpgamd.CpgError, match="cost less than its parent"):
+        g.report_paths(k, root_tie_quirk=False, monotone_check=True)
+
+
+def test_presets_have_no_nonmonotone_children():
+    """The counter is 0 on the K=10000 query of the 50K-vertex DAG, and the
+    check passes there."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1)
+    g = engine_for(n, u, v, w)
+    g.report_paths(10000, monotone_check=True)
+    assert g.timings()["nonmonotone"] == 0
+
+
+def test_mem_budget_bytes_bounds_the_node_store():
+    """mem_budget_bytes: a budget with room for the store gives the exact
+    answer within it; one with no room, or one the store must outgrow after
+    an overflow, fails with an error naming the budget."""
+    n, u, v, w, og, dist, succ, level, L = preset_graph(1)
+    g = engine_for(n, u, v, w)
+    g.report_paths(10)
+    t = g.timings()
+    sfxt_bytes = t["device_bytes"] - 61 * t["node_slots"]  # graph + SFXT
+    say(f"case C1 memory budget (graph + SFXT {sfxt_bytes} bytes, store {t['node_slots']} slots)")
+    want, _ = og.topk(dist, succ, 10000)
+    budget = sfxt_bytes + 61 * 400_000
+    assert budget < t["device_bytes"]
+    g.report_paths(10000, mem_budget_bytes=budget)
+    assert_bits_equal(want, g.get_slacks(10000), "C1 K=10000 within the budget")
+    assert g.timings()["device_bytes"] <= budget and g.timings()["node_slots"] <= 400_000
+    with pytest.raises(cpgamd.CpgError, match="mem_budget_bytes"):
+        g.report_paths(10000, mem_budget_bytes=sfxt_bytes + 61 * 100)
+    # 60000 slots overflow at K=50000 (see the overflow test above); the
+    # budget lets the store grow to 61000 slots only
+    with pytest.raises(cpgamd.CpgError, match="mem_budget_bytes"):
+        g.report_paths(50000, max_nodes=60000, mem_budget_bytes=sfxt_bytes + 61 * 61000)
+    g.report_paths(50000, max_nodes=60000)
+    assert g.timings()["retries"] >= 1
