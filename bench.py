@@ -5,7 +5,8 @@ and 300 levels (out-degree geometric, mean 2.7; lognormal(-2, 0.7) delays;
 2% sources, 2% sinks), one query for the K = 1,000,000 lowest-cost paths.
 A step is one complete query with the graph already resident in HBM:
 SFXT (levelize, relax, successor, deviation index) + PFXT expansion + final
-top-K selection, results left in HBM.  The static cache is off, so every
+top-K selection, and the K costs copied to host memory (get_slacks).  The
+static cache is off, so every
 step recomputes the SFXT.
 
 Prints one JSON line.  Besides the contract fields it carries
@@ -49,6 +50,26 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def measured_traffic(kernel):
@@ -276,7 +297,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--k", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-trials", type=int, default=2)
+    ap.add_argument("--cpu-trials", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-bvss", action="store_true")
@@ -303,15 +324,22 @@ def main():
         g.report_paths(k, kernel_timing=kernel_timing)
         return g.timings()
 
-    for i in range(args.warmup):
+    def step():
+        """One query with its K costs copied to host memory, as the
+        reference's report_paths + get_slacks hand them over."""
         t = run()
+        g.get_slacks(k)
+        return t
+
+    for i in range(args.warmup):
+        t = step()
         log("warmup %d: %.3f ms" % (i, t["query_ms"]))
     step_ms = []
     tims = []
     t_start = time.perf_counter()
     for i in range(args.steps):
         s0 = time.perf_counter()
-        t = run()
+        t = step()
         step_ms.append((time.perf_counter() - s0) * 1e3)
         tims.append(t)
     total_s = time.perf_counter() - t_start
@@ -340,7 +368,7 @@ def main():
     walk_bytes = 16.0 * st["hops"] + 16.0 * st["dev_visits"] + 12.0 * st["expanded"] \
         + 25.0 * (st["generated"] - st["roots"])
     kernels = {
-        "sfxt_levels_kernel": (levels_ms, levels_bytes, 1),
+        "sfxt_levels_nf_kernel": (levels_ms, levels_bytes, 1),
         "walk_kernel": (walk_ms, walk_bytes, max(walk_launches, 1)),
     }
     dom = max(kernels, key=lambda name: kernels[name][0])
@@ -403,12 +431,15 @@ def main():
             total, s_ms, p_ms, cpu_costs = og.query_ms(k)
             log("cpu trial %d: %.1f ms (sfxt %.1f, pfxt %.1f)" % (trial, total, s_ms, p_ms))
             if trial > 0:
-                res.append(total)
+                res.append((total, s_ms, p_ms))
         out["cpu_baseline"] = {
-            "value": round(statistics.mean(res), 2), "unit": "ms", "cores": 1, "kind": "port",
+            "value": round(statistics.mean(r[0] for r in res), 2), "unit": "ms", "cores": 1,
+            "kind": "port",
             "sample": "full query on the same graph and K (host SFXT + K-capped best-first PFXT),"
                       " single thread, 1 warmup + %d trials" % args.cpu_trials,
-            "nproc": os.cpu_count()}
+            "sfxt_ms": round(statistics.mean(r[1] for r in res), 2),
+            "pfxt_ms": round(statistics.mean(r[2] for r in res), 2),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
         out["exact"] = bool(len(cpu_costs) == len(costs)
                             and pyoracle.first_bit_mismatch(cpu_costs, costs) == -1)
     g.close()  # every engine object is closed before exit, none from a finalizer

@@ -50,7 +50,10 @@ struct SfxtArgs {
   SfxtCtl* ctl;
   unsigned long long* dfw;  // dataflow loop: one progress word per block (dfw_n slots)
   int dfw_n;
-  int pad;
+  // level loops: only blocks whose index is a multiple of place_stride take
+  // part (the others return at once); with a stride of 8 (the XCD count)
+  // every working block lands on the same XCD under round-robin dispatch
+  int place_stride;
 };
 
 // Q endpoint subsets relaxed in one level loop (served queries): the
@@ -93,11 +96,19 @@ void launch_sfxt_levels_parts(const SfxtParts& parts, int total_blocks, hipStrea
 // Per-vertex init, then the cooperative level loop (returns the grid used).
 void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);
+// The same loop without cache fences (device-scope queue and best[] access);
+// the queue must hold the sinks and -1 beyond: launch_sfxt_dataflow_init
+// with args.dfw_n = 0 prepares it.
+int launch_sfxt_levels_nf(const SfxtArgs& args, int blocks, hipStream_t stream);
 // The barrier-free dataflow loop (same results as the level loop); args.dfw
 // must hold at least sfxt_dataflow_grid(blocks) words.  Returns the grid used.
 int sfxt_dataflow_grid(int blocks);
 void launch_sfxt_dataflow_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_dataflow(const SfxtArgs& args, int blocks, hipStream_t stream);
+// The local-queue level loop (block-local LDS queues, fence-free grid
+// barrier; same results as the level loop).  Returns the grid used.
+void launch_sfxt_local_init(const SfxtArgs& args, hipStream_t stream);
+int launch_sfxt_local(const SfxtArgs& args, int blocks, hipStream_t stream);
 // Deviation records, their sort, the lift tables and the nsv pointers.
 // m: the graph's edge count (chooses thread- or wave-per-vertex records).
 void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream);

@@ -11,9 +11,10 @@
 // Every fanout of a level-d vertex sits on a lower level, so relax and
 // successor for level d run in the same pass that peels level d.  That
 // pass is the only part with a level-to-level dependency; it runs as one
-// cooperative kernel on a small grid (the per-level grid barrier costs more
-// with more blocks, and levels hold a few thousand vertices) and does no
-// more than the relax, the successor and the fanin release.
+// resident kernel with a grid barrier per level on a small grid (the
+// barrier costs more with more blocks, and levels hold a few thousand
+// vertices) and does no more than the relax, the successor and the fanin
+// release.
 //
 // Endpoint subset (an extension; the reference has no such option): with a
 // sink mask, only the chosen sinks start at distance 0.  The other sinks,
@@ -62,6 +63,23 @@ constexpr int kSortBlock = 256;
 __device__ inline unsigned long long atomic_load_rlx64(unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Device-scope (relaxed) loads and stores of words other blocks write or
+// read during the same kernel.
+__device__ inline unsigned long long ld_dev64(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_dev32(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev32(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned ld_devu32(unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Generation barrier over the blocks of one level loop, which also hands
 // every block the queue tail of the next level.  One release fence before
@@ -78,8 +96,15 @@ __device__ inline unsigned long long atomic_load_rlx64(unsigned long long* p) {
 // already pushing the next level.)  A spin that exceeds kSpinLimit raises
 // ctl->err and every later barrier returns false at once, so the kernel
 // always drains.  Returns false on error.
+//
+// kFenced = false: the fence-free variant for loops whose cross-block data
+// (queue slots, best[] words) is itself written and read at device scope.
+// A block then arrives once its own stores and atomics of the level have
+// completed (s_waitcnt), and no block writes back or invalidates its L2.
+template <bool kFenced = true>
 __device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned pushed, unsigned le,
                              unsigned* next_tail) {
+  if (!kFenced) __builtin_amdgcn_s_waitcnt(0);  // this thread's stores and atomics are done
   __syncthreads();
   __shared__ int s_ok;
   __shared__ unsigned s_tail;
@@ -87,15 +112,21 @@ __device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned pushed, un
     int ok = 1;
     unsigned tail = 0;
     const unsigned g = static_cast<unsigned>(atomic_load_rlx64(&ctl->bar_word) >> 32);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (kFenced) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned long long old = __hip_atomic_fetch_add(
         &ctl->bar_arrive, (1ull << 40) | pushed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (static_cast<unsigned>(old >> 40) == nblocks - 1) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (kFenced) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       tail = le + static_cast<unsigned>(old & ((1ull << 40) - 1)) + pushed;
       __hip_atomic_store(&ctl->bar_arrive, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->bar_word, (static_cast<unsigned long long>(g + 1u) << 32) | tail,
-                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (kFenced) {
+        __hip_atomic_store(&ctl->bar_word, (static_cast<unsigned long long>(g + 1u) << 32) | tail,
+                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __builtin_amdgcn_s_waitcnt(0);  // the reset lands before any block can arrive again
+        __hip_atomic_store(&ctl->bar_word, (static_cast<unsigned long long>(g + 1u) << 32) | tail,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     } else {
       unsigned spins = 0;
       for (;;) {
@@ -109,7 +140,7 @@ __device__ bool grid_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned pushed, un
         __builtin_amdgcn_s_sleep(1);
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (kFenced) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
     s_tail = tail;
     s_ok = ok;
@@ -173,21 +204,30 @@ __global__ void __launch_bounds__(256) sfxt_init_kernel(SfxtArgs a) {
 // round trips (queue entry; best and fanin range; fanin records; the
 // atomics), with all loads of a round issued together (kFaninUnroll fanins
 // per step).
-__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
+//
+// kFenced = false (the fence-free variant): queue slots are written with
+// device-scope stores and read with device-scope loads (-1 until written),
+// best[] is read with device-scope loads, and the grid barrier has no
+// release/acquire fence; the queue must be initialised to the sinks and -1.
+template <bool kFenced>
+__device__ void levels_loop(const SfxtArgs& a) {
   __shared__ int s_q[kLdsQueue];
   __shared__ int s_qn;
   __shared__ unsigned s_base;
   SfxtCtl* ctl = a.ctl;
-  const unsigned nblocks = gridDim.x;
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int gthreads = gridDim.x * blockDim.x;
+  const unsigned stride = a.place_stride > 1 ? static_cast<unsigned>(a.place_stride) : 1u;
+  if (blockIdx.x % stride != 0) return;  // block-uniform: not a working block
+  const unsigned nblocks = gridDim.x / stride;
+  const int gtid = static_cast<int>(blockIdx.x / stride * blockDim.x + threadIdx.x);
+  const int gthreads = static_cast<int>(nblocks * blockDim.x);
   if (threadIdx.x == 0) s_qn = 0;
   __syncthreads();
 
   auto push = [&](int x) {
     const int p = atomicAdd(&s_qn, 1);
     if (p < kLdsQueue) s_q[p] = x;
-    else a.queue[atomicAdd(&ctl->qtail, 1u)] = x;
+    else if (kFenced) a.queue[atomicAdd(&ctl->qtail, 1u)] = x;
+    else st_dev32(a.queue + atomicAdd(&ctl->qtail, 1u), x);
   };
   // Writes the block's LDS queue out; returns every vertex the block
   // pushed this level (those that spilled past the LDS queue included).
@@ -197,7 +237,10 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
     const int cnt = min(total, kLdsQueue);
     if (threadIdx.x == 0) s_base = cnt ? atomicAdd(&ctl->qtail, static_cast<unsigned>(cnt)) : 0u;
     __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) a.queue[s_base + i] = s_q[i];
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+      if (kFenced) a.queue[s_base + i] = s_q[i];
+      else st_dev32(a.queue + s_base + i, s_q[i]);
+    }
     __syncthreads();
     if (threadIdx.x == 0) s_qn = 0;
     return static_cast<unsigned>(total);
@@ -209,8 +252,22 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
   while (true) {
     if (gtid == 0) a.lvlp[d] = static_cast<int>(lb);
     for (unsigned i = lb + gtid; i < le; i += gthreads) {
-      const int u = a.queue[i];
-      const unsigned long long key = a.best[u];
+      int u;
+      unsigned long long key;
+      if (kFenced) {
+        u = a.queue[i];
+        key = a.best[u];
+      } else {
+        u = ld_dev32(a.queue + i);
+        unsigned spins = 0;
+        while (u < 0) {  // written before the barrier; a store still landing
+          if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); break; }
+          __builtin_amdgcn_s_sleep(1);
+          u = ld_dev32(a.queue + i);
+        }
+        if (u < 0) continue;
+        key = ld_dev64(a.best + u);
+      }
       const int b = a.iadjp[u], e = a.iadjp[u + 1];
       // level 0: a sink outside the endpoint subset is unreachable; later
       // levels: no candidate arrived (every fanout unreachable)
@@ -253,7 +310,7 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
     }
     const unsigned pushed = flush();
     unsigned ne = 0;
-    if (!grid_barrier(ctl, nblocks, pushed, le, &ne)) return;
+    if (!grid_barrier<kFenced>(ctl, nblocks, pushed, le, &ne)) return;
     if (ne == le) break;
     lb = le;
     le = ne;
@@ -267,6 +324,9 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) {
     ctl->levelized = le;
   }
 }
+
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) { levels_loop<true>(a); }
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf_kernel(SfxtArgs a) { levels_loop<false>(a); }
 
 // ---------------------------------------------------------------------------
 // Dataflow loop: the same levelize / relax / successor results as the level
@@ -311,21 +371,6 @@ constexpr int kDfKeep = 512;    // pool entries kept after an iteration
 constexpr int kDfUnroll = 8;    // edge records (and their loads / atomics) in flight per step
 constexpr unsigned long long kStateEmpty = ~0ull;
 
-__device__ inline unsigned long long ld_dev64(unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_dev64(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline int ld_dev32(int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_dev32(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline unsigned ld_devu32(unsigned* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ inline int wave_max_int(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
   return v;
@@ -586,6 +631,241 @@ __global__ void __launch_bounds__(kDfBlock) sfxt_dataflow_kernel(SfxtArgs a) {
   if (t == 0) atomicAdd(&ctl->levelized, my_p);
 }
 
+// ---------------------------------------------------------------------------
+// Local-queue level loop: the level loop's levels, dist and succ with no
+// queue flush, no global queue read and no cache maintenance per level.
+//
+// A block settles the vertices it released itself during the previous
+// level.  They wait in an LDS queue together with their fanin ranges, which
+// the releasing thread loads beside its release atomics, so settling a
+// vertex starts with its best[] word and its fanin records in one round
+// trip.  Releases past the LDS queue spill to a global queue that every
+// block shares out at the next level (slots -1 until written).  Level 0 is
+// the sink list, shared out the same way.
+//
+// Everything one block reads of another block's work is read at device
+// scope: best[] with device-scope atomic loads (the candidates arrive by
+// atomicMin, a read-modify-write performed at the device's coherence
+// point), spill slots with device-scope loads of device-scope stores.  A
+// block arrives at the level's barrier once its own stores and atomics of
+// the level have completed (s_waitcnt), and every read of the next level
+// comes after the barrier.  So the barrier needs no release or acquire
+// fence: the L2s keep the read-only CSR lines from level to level instead
+// of writing back and invalidating at every level.
+//
+// The barrier's arrival word counts blocks in bits 56-63, sums the blocks'
+// releases in bits 28-55 and their spills in bits 0-27 (graphs of fewer
+// than 2^28 vertices).  The last block to arrive publishes the next
+// generation with the level's spill count, bit 31 set when nothing was
+// released (the loop ends).  Spills of level d+1 are exactly the slots
+// [end of level d's spills, + count): they are all appended during level d.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kLocBlock = 256;
+constexpr int kLocCap = 1536;   // LDS queue entries per level buffer
+constexpr int kLocUnroll = 8;   // fanin records (loads and atomics) in flight per step
+constexpr unsigned kLocMask = (1u << 28) - 1;
+
+// Returns false when the spin limit was exceeded or another block raised
+// the error flag; *spills = the next level's spill count, *more = whether
+// any block released a vertex.
+__device__ bool local_barrier(SfxtCtl* ctl, unsigned nblocks, unsigned pushed, unsigned spilled,
+                              unsigned* spills, bool* more) {
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's stores and atomics of the level are done
+  __syncthreads();
+  __shared__ int s_ok;
+  __shared__ unsigned s_low;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    unsigned low = 0;
+    const unsigned g = static_cast<unsigned>(ld_dev64(&ctl->bar_word) >> 32);
+    const unsigned long long add = (1ull << 56) | (static_cast<unsigned long long>(pushed) << 28) | spilled;
+    const unsigned long long old = __hip_atomic_fetch_add(&ctl->bar_arrive, add, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+    if (static_cast<unsigned>(old >> 56) == nblocks - 1) {
+      const unsigned long long all = old + add;
+      const unsigned tp = static_cast<unsigned>(all >> 28) & kLocMask;
+      low = (static_cast<unsigned>(all) & kLocMask) | (tp == 0 ? 0x80000000u : 0u);
+      st_dev64(&ctl->bar_arrive, 0ull);
+      __builtin_amdgcn_s_waitcnt(0);  // the reset lands before any block can arrive again
+      st_dev64(&ctl->bar_word, (static_cast<unsigned long long>(g + 1u) << 32) | low);
+    } else {
+      unsigned spins = 0;
+      for (;;) {
+        const unsigned long long w = ld_dev64(&ctl->bar_word);
+        if (static_cast<unsigned>(w >> 32) != g) {
+          low = static_cast<unsigned>(w);
+          break;
+        }
+        if (ld_dev32(&ctl->err) != 0) { ok = 0; break; }
+        if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); ok = 0; break; }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (ld_dev32(&ctl->err) != 0) ok = 0;
+    s_low = low;
+    s_ok = ok;
+  }
+  __syncthreads();
+  *spills = s_low & kLocMask;
+  *more = (s_low & 0x80000000u) == 0;
+  return s_ok != 0;
+}
+}  // namespace
+
+// Per-vertex state before the local-queue loop; spill slots start at -1.
+__global__ void __launch_bounds__(256) sfxt_local_init_kernel(SfxtArgs a) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gthreads = gridDim.x * blockDim.x;
+  for (int v = gtid; v < a.n; v += gthreads) {
+    a.deps[v] = a.oadjp[v + 1] - a.oadjp[v];
+    a.level[v] = -1;
+    a.dist[v] = kIntMax;
+    a.succ[v] = -1;
+    a.distf[v] = static_cast<float>(kIntMax) / static_cast<float>(kScaleUp);
+    a.best[v] = ~0ull;
+    a.queue[v] = -1;
+  }
+}
+
+__global__ void __launch_bounds__(kLocBlock) sfxt_local_kernel(SfxtArgs a) {
+  __shared__ int s_x[2][kLocCap];
+  __shared__ int s_b[2][kLocCap];
+  __shared__ int s_e[2][kLocCap];
+  __shared__ int s_n[2];
+  __shared__ unsigned s_done;
+  SfxtCtl* ctl = a.ctl;
+  const unsigned G = gridDim.x;
+  const unsigned bid = blockIdx.x;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    s_n[0] = 0;
+    s_n[1] = 0;
+    s_done = 0;
+  }
+  __syncthreads();
+  unsigned sp_lo = 0, sp_n = 0;  // this level's spill slots [sp_lo, sp_lo + sp_n)
+  int d = 0;
+  unsigned settled = 0;
+  float local_max = 0.0f;
+  bool ok = true;
+  while (true) {
+    const int cur = d & 1, nxt = cur ^ 1;
+    if (t == 0) s_n[nxt] = 0;
+    __syncthreads();
+    unsigned nloc, q0, q1;
+    if (d == 0) {
+      nloc = 0;
+      q0 = static_cast<unsigned>(static_cast<unsigned long long>(a.nsinks) * bid / G);
+      q1 = static_cast<unsigned>(static_cast<unsigned long long>(a.nsinks) * (bid + 1) / G);
+    } else {
+      nloc = static_cast<unsigned>(min(s_n[cur], kLocCap));
+      q0 = sp_lo + static_cast<unsigned>(static_cast<unsigned long long>(sp_n) * bid / G);
+      q1 = sp_lo + static_cast<unsigned>(static_cast<unsigned long long>(sp_n) * (bid + 1) / G);
+    }
+    const unsigned items = nloc + (q1 - q0);
+
+    auto push = [&](int x, int xb, int xe) {
+      const int p = atomicAdd(&s_n[nxt], 1);
+      if (p < kLocCap) {
+        s_x[nxt][p] = x;
+        s_b[nxt][p] = xb;
+        s_e[nxt][p] = xe;
+      } else {
+        const unsigned q = atomicAdd(&ctl->qtail, 1u);
+        st_dev32(a.queue + q, x);
+      }
+    };
+
+    for (unsigned i = static_cast<unsigned>(t); i < items; i += kLocBlock) {
+      int u, b, e;
+      if (i < nloc) {
+        u = s_x[cur][i];
+        b = s_b[cur][i];
+        e = s_e[cur][i];
+      } else {
+        const unsigned qi = q0 + (i - nloc);
+        if (d == 0) {
+          u = a.sinks[qi];
+        } else {
+          u = ld_dev32(a.queue + qi);
+          unsigned spins = 0;
+          while (u < 0) {  // written before the barrier; a store still landing
+            if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); break; }
+            __builtin_amdgcn_s_sleep(1);
+            u = ld_dev32(a.queue + qi);
+          }
+          if (u < 0) continue;
+        }
+        b = a.iadjp[u];
+        e = a.iadjp[u + 1];
+      }
+      int du = 0, s = -1;
+      if (d == 0) {
+        if (a.sink_mask && !a.sink_mask[u]) du = kIntMax;
+      } else {
+        const unsigned long long key = ld_dev64(a.best + u);
+        if (key == ~0ull) {
+          du = kIntMax;
+        } else {
+          du = unpack_cand(key);
+          s = unpack_succ(key);
+        }
+      }
+      const bool reach = du != kIntMax;
+      const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
+      a.dist[u] = du;
+      a.succ[u] = s;
+      a.level[u] = d;
+      a.distf[u] = df;
+      ++settled;
+      if (reach) local_max = fmaxf(local_max, fabsf(df));
+      for (int j = b; j < e; j += kLocUnroll) {
+        // kLocUnroll fanin records; past the end the last one is loaded
+        // again (every load unconditional) and its atomics are skipped.  The
+        // fanin's CSR range is loaded beside the atomics, for the LDS queue.
+        int2 r[kLocUnroll];
+#pragma unroll
+        for (int q = 0; q < kLocUnroll; ++q) r[q] = a.iedge[min(j + q, e - 1)];
+        int xb[kLocUnroll], xe[kLocUnroll], old[kLocUnroll];
+#pragma unroll
+        for (int q = 0; q < kLocUnroll; ++q) {
+          xb[q] = a.iadjp[r[q].x];
+          xe[q] = a.iadjp[r[q].x + 1];
+          old[q] = 0;
+          if (j + q < e) {
+            const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
+            if (reach) atomicMin(&a.best[r[q].x], pack_cand(du + wi, u));
+            old[q] = atomicSub(&a.deps[r[q].x], 1);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kLocUnroll; ++q)
+          if (j + q < e && old[q] == 1) push(r[q].x, xb[q], xe[q]);
+      }
+    }
+    __syncthreads();
+    const int pushed = s_n[nxt];
+    const unsigned spilled = pushed > kLocCap ? static_cast<unsigned>(pushed - kLocCap) : 0u;
+    unsigned next_spills = 0;
+    bool more = false;
+    if (!local_barrier(ctl, G, static_cast<unsigned>(pushed), spilled, &next_spills, &more)) {
+      ok = false;
+      break;
+    }
+    if (!more) break;
+    sp_lo += sp_n;
+    sp_n = next_spills;
+    ++d;
+  }
+  const float m = wave_max(local_max);
+  if (__lane_id() == 0) atomicMax(&ctl->max_abs_distf_bits, __float_as_uint(m));
+  atomicAdd(&s_done, settled);
+  __syncthreads();
+  if (t == 0) atomicAdd(&ctl->levelized, s_done);
+  if (ok && bid == 0 && t == 0) ctl->num_levels = a.nsinks > 0 ? d + 1 : 0;
+}
+
 // Per-vertex state before the multi-subset level loop.
 __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMulti m) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -720,7 +1000,7 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
   }
 }
 
-// Level loops of several graphs in one cooperative launch: each block
+// Level loops of several graphs in one resident launch: each block
 // runs the loop of the partition its index falls in, and a partition's
 // barrier counts only its own blocks, so every graph advances at its own
 // pace while all of them are resident together.
@@ -1014,6 +1294,29 @@ int flat_grid(int n, int cus) {
   return std::max(1, std::min((n + 255) / 256, cus * 8));
 }
 
+// Kernels with a grid barrier launch as ordinary kernels on a grid of at
+// most half the blocks the device keeps resident at once, so every block
+// of the grid is dispatched at once even beside another kernel of this
+// process.  (A cooperative launch would promise the same, but on this
+// platform it fails with "cannot be performed in the present state" while
+// another process on the GPU holds the cooperative-launch resources, e.g. a
+// driver started from a process that has made cooperative launches.)  A
+// barrier wait that never completes ends at its spin limit with an error,
+// never in a hang.
+template <typename K, typename A>
+void launch_resident(K kernel, int grid, int block, const A& args, hipStream_t stream) {
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, args);
+  CPG_HIP(hipGetLastError());
+}
+
+template <typename K>
+int resident_blocks(K kernel, int block) {
+  int per_cu = 0;
+  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+  if (per_cu < 1) throw std::runtime_error("grid-barrier kernel cannot be resident");
+  return std::max(1, device_cus() * per_cu / 2);
+}
+
 }  // namespace
 
 int sfxt_lift_levels(int num_levels) {
@@ -1028,21 +1331,25 @@ void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream) {
   CPG_HIP(hipGetLastError());
 }
 
+// Working blocks of a level loop and its launch grid (place_stride blocks
+// per working block).
+static int placed_grid(int blocks, int resident, int stride) {
+  const int working = std::max(1, std::min(blocks, resident / std::max(1, stride)));
+  return working * std::max(1, stride);
+}
+
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
-  int dev = 0, coop = 0;
-  CPG_HIP(hipGetDevice(&dev));
-  CPG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-  if (!coop) throw std::runtime_error("device does not support cooperative launch");
-  const int cus = device_cus();
-  int per_cu = 0;
-  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_levels_kernel, kSfxtBlock, 0));
-  if (per_cu < 1) throw std::runtime_error("sfxt kernel cannot be resident");
-  const int grid = std::max(1, std::min(blocks, cus * per_cu));
-  SfxtArgs a = args;
-  void* params[] = {&a};
-  CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_kernel, dim3(grid), dim3(kSfxtBlock), params, 0,
-                                     stream));
-  return grid;
+  const int stride = std::max(1, args.place_stride);
+  const int grid = placed_grid(blocks, resident_blocks(sfxt_levels_kernel, kSfxtBlock), stride);
+  launch_resident(sfxt_levels_kernel, grid, kSfxtBlock, args, stream);
+  return grid / stride;
+}
+
+int launch_sfxt_levels_nf(const SfxtArgs& args, int blocks, hipStream_t stream) {
+  const int stride = std::max(1, args.place_stride);
+  const int grid = placed_grid(blocks, resident_blocks(sfxt_levels_nf_kernel, kSfxtBlock), stride);
+  launch_resident(sfxt_levels_nf_kernel, grid, kSfxtBlock, args, stream);
+  return grid / stride;
 }
 
 int sfxt_dataflow_grid(int blocks) {
@@ -1061,17 +1368,30 @@ void launch_sfxt_dataflow_init(const SfxtArgs& args, hipStream_t stream) {
   CPG_HIP(hipGetLastError());
 }
 
+// An ordinary launch: no block waits for a block that has not started (a
+// wait on a fanout's state word ends once that vertex's store lands, and
+// an idle block leaves at quiescence), so the loop needs no co-residency
+// guarantee.  A cooperative launch would also tie up the device's
+// cooperative-launch resources, which another process on the same GPU may
+// hold.
 int launch_sfxt_dataflow(const SfxtArgs& args, int blocks, hipStream_t stream) {
-  int dev = 0, coop = 0;
-  CPG_HIP(hipGetDevice(&dev));
-  CPG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-  if (!coop) throw std::runtime_error("device does not support cooperative launch");
   const int grid = sfxt_dataflow_grid(blocks);
   if (args.dfw_n < grid || !args.dfw) throw std::runtime_error("dataflow progress words too few");
-  SfxtArgs a = args;
-  void* params[] = {&a};
-  CPG_HIP(hipLaunchCooperativeKernel(sfxt_dataflow_kernel, dim3(grid), dim3(kDfBlock), params, 0,
-                                     stream));
+  hipLaunchKernelGGL(sfxt_dataflow_kernel, dim3(grid), dim3(kDfBlock), 0, stream, args);
+  CPG_HIP(hipGetLastError());
+  return grid;
+}
+
+void launch_sfxt_local_init(const SfxtArgs& args, hipStream_t stream) {
+  hipLaunchKernelGGL(sfxt_local_init_kernel, dim3(flat_grid(args.n, device_cus())), dim3(256), 0,
+                     stream, args);
+  CPG_HIP(hipGetLastError());
+}
+
+int launch_sfxt_local(const SfxtArgs& args, int blocks, hipStream_t stream) {
+  if (args.n >= (1 << 28)) throw std::runtime_error("local-queue level loop: graph too large");
+  const int grid = std::max(1, std::min({blocks, resident_blocks(sfxt_local_kernel, kLocBlock), 255}));
+  launch_resident(sfxt_local_kernel, grid, kLocBlock, args, stream);
   return grid;
 }
 
@@ -1081,18 +1401,9 @@ void launch_sfxt_init_multi(const SfxtArgs& args, const SfxtMulti& m, hipStream_
   CPG_HIP(hipGetLastError());
 }
 
-int sfxt_parts_capacity() {
-  int per_cu = 0;
-  CPG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sfxt_levels_parts_kernel, kSfxtBlock, 0));
-  if (per_cu < 1) throw std::runtime_error("sfxt kernel cannot be resident");
-  return device_cus() * per_cu;
-}
+int sfxt_parts_capacity() { return resident_blocks(sfxt_levels_parts_kernel, kSfxtBlock); }
 
 void launch_sfxt_levels_parts(const SfxtParts& parts, int total_blocks, hipStream_t stream) {
-  int dev = 0, coop = 0;
-  CPG_HIP(hipGetDevice(&dev));
-  CPG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-  if (!coop) throw std::runtime_error("device does not support cooperative launch");
   if (parts.n < 1 || parts.n > kMaxParts || total_blocks > sfxt_parts_capacity())
     throw std::runtime_error("level-loop partitions do not fit one resident launch");
   for (int i = 0; i < parts.n; ++i) {
@@ -1100,10 +1411,7 @@ void launch_sfxt_levels_parts(const SfxtParts& parts, int total_blocks, hipStrea
     if (parts.p[i].nb < 1 || parts.p[i].b0 + parts.p[i].nb != end || (i == 0 && parts.p[0].b0 != 0))
       throw std::runtime_error("level-loop partitions must tile the launch");
   }
-  SfxtParts pp = parts;
-  void* params[] = {&pp};
-  CPG_HIP(hipLaunchCooperativeKernel(sfxt_levels_parts_kernel, dim3(total_blocks), dim3(kSfxtBlock),
-                                     params, 0, stream));
+  launch_resident(sfxt_levels_parts_kernel, total_blocks, kSfxtBlock, parts, stream);
 }
 
 void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream) {

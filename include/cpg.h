@@ -59,9 +59,13 @@ typedef struct {
   int bound_probe;          /* first bound from pruned count passes: 0 = when the
                                roots' chains are long (automatic), 1 = always,
                                -1 = never (the first round walks unbounded)   */
-  int sfxt_method;          /* 0: barrier-free dataflow loop (default), 1: the
-                               level-synchronous loop (one grid barrier per
-                               level); both give identical results            */
+  int sfxt_method;          /* SFXT loop: 0 automatic (4 with its blocks on one
+                               XCD), 1 level-synchronous loop with a global
+                               queue and a fenced grid barrier, 2 barrier-free
+                               dataflow loop, 3 level-synchronous loop with
+                               block-local queues, 4 loop 1 with device-scope
+                               queue access and a fence-free barrier; all give
+                               identical results                              */
   int step_times;           /* 1: record every expansion round's device time and
                                the paths it stored (cpg_step_times)           */
   int monotone_check;       /* 1: fail the query (error) when a child's binary32

@@ -188,6 +188,9 @@ inline T atomicMin(T* p, T v) {
 #define __hip_atomic_fetch_add(p, v, order, scope) __atomic_fetch_add((p), (v), (order))
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 #define __builtin_amdgcn_s_sleep(n) simt::relax()
+// waiting for a thread's outstanding memory operations: on the CPU they are
+// done once issued, so only the compiler's ordering is kept
+#define __builtin_amdgcn_s_waitcnt(n) __atomic_thread_fence(__ATOMIC_SEQ_CST)
 
 // ---------------------------------------------------------------- host API
 typedef int hipError_t;

@@ -227,6 +227,15 @@ void relax() { std::this_thread::sleep_for(std::chrono::microseconds(20)); }
 
 void run_grid(const std::function<void()>& body, dim3 grid, dim3 block, bool cooperative) {
   const unsigned nb = grid.x * grid.y * grid.z;
+  // A GPU dispatches every block of a small grid at once, and the engine's
+  // grid-barrier kernels rely on that (they launch ordinary grids within
+  // the device's resident capacity): grids of up to SIMT_RESIDENT blocks
+  // (default 16) run one OS thread per block, as cooperative launches do.
+  static const unsigned resident = [] {
+    const char* e = std::getenv("SIMT_RESIDENT");
+    return e ? static_cast<unsigned>(std::max(1, std::atoi(e))) : 16u;
+  }();
+  if (!cooperative && nb <= resident) cooperative = true;
   if (!cooperative) {
     // SIMT_THREADS > 1 spreads the blocks of ordinary launches over that
     // many OS threads, so that blocks run concurrently as they do on a GPU

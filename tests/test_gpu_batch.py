@@ -69,3 +69,31 @@ def test_batch_result_equals_plain_query_and_rerun():
         assert pyoracle.first_bit_mismatch(plain, first[q]) == -1 and len(plain) == len(first[q])
         again = b.get_slacks(q, k)
         assert pyoracle.first_bit_mismatch(first[q], again) == -1 and len(again) == len(first[q])
+
+
+def test_full_256_design_batch_equals_oracle():
+    """The whole 256-design batch workload (5K-200K vertices each, K
+    100-100K) in one Batch: every query equals the oracle's top-K of its
+    design alone, bit for bit."""
+    designs = []
+    for i in range(256):
+        n, u, v, w = cpgamd.gen_preset(4, i)
+        designs.append((n, u, v, w, int(cpgamd.preset_k(4, i))))
+    print(f"case full batch of {len(designs)} designs", flush=True)
+    b = cpgamd.Batch()
+    try:
+        for n, u, v, w, k in designs:
+            b.add(n, u, v, w, k)
+        b.run()
+        print(f"batch timings {b.timings()}", flush=True)
+        bad = []
+        for q, (n, u, v, w, k) in enumerate(designs):
+            og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+            dist, succ, _, _ = og.sfxt()
+            want, _ = og.topk(dist, succ, k)
+            got = b.get_slacks(q, k)
+            if len(got) != len(want) or pyoracle.first_bit_mismatch(want, got) != -1:
+                bad.append((q, len(got), len(want), pyoracle.first_bit_mismatch(want, got)))
+        assert not bad, bad[:8]
+    finally:
+        b.close()

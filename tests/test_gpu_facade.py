@@ -35,7 +35,11 @@ def test_cpp_facade_query_matches_oracle(tmp_path, k):
     assert len(got) == len(want) and pyoracle.first_bit_mismatch(want, got) == -1
     steps = int(vals["steps"])
     assert steps >= 1 and int(vals["step_times"]) == steps and int(vals["paths_per_step"]) == steps
-    assert 0 < int(vals["gen"]) <= int(vals["total"])
+    # paths stored by the rounds (K=10: the roots and the bound probe's
+    # children already hold the top 10, so the one round may store none)
+    assert 0 <= int(vals["gen"]) <= int(vals["total"])
+    if k >= 10000:
+        assert int(vals["gen"]) > 0
     assert float(vals["pfxt_ms"]) > 0.0 and float(vals["lvlize_ms"]) > 0.0
     assert float(vals["pfxt_ms"]) <= float(vals["expand_ms"]) + 1e-6
     assert int(vals["diameter"]) == L

@@ -345,14 +345,16 @@ def star_graph(fan=5000, seed=5):
     return n, np.array(u, np.int32), np.array(v, np.int32), w
 
 
-@pytest.mark.parametrize("blocks", [1, 2, 0])
-def test_sfxt_dataflow_pool_overflow(blocks):
+@pytest.mark.parametrize("method,blocks", [(2, 1), (2, 2), (2, 0), (3, 1), (3, 2), (3, 0)])
+def test_sfxt_lds_queue_overflow(method, blocks):
+    """The dataflow loop's LDS pool and the local-queue loop's LDS level
+    buffer overflow into their global queues; results are unchanged."""
     n, u, v, w = star_graph()
-    say(f"case star graph dataflow blocks={blocks}")
+    say(f"case star graph sfxt method {method} blocks={blocks}")
     og = pyoracle.OracleGraph.from_edges(n, u, v, w)
     dist, succ, level, L = og.sfxt()
     g = engine_for(n, u, v, w)
-    g.report_paths(1000, sfxt_blocks=blocks)
+    g.report_paths(1000, sfxt_blocks=blocks, sfxt_method=method)
     gd, gs, gl = g.sfxt()
     np.testing.assert_array_equal(gd, dist)
     np.testing.assert_array_equal(gs, succ)
@@ -364,11 +366,12 @@ def test_sfxt_dataflow_pool_overflow(blocks):
 
 @pytest.mark.parametrize("which", [1, 2, 3])
 def test_sfxt_methods_agree(which):
-    """The dataflow loop (default) and the level-synchronous loop give the
-    oracle's dist/succ/level on C1, C2 and the densified C3."""
+    """The four SFXT loops (level loop with a global queue, dataflow loop,
+    local-queue level loop, fence-free level loop) give the oracle's
+    dist/succ/level on C1, C2 and the densified C3."""
     n, u, v, w, og, dist, succ, level, L = preset_graph(which)
     g = engine_for(n, u, v, w)
-    for method in (0, 1):
+    for method in (1, 2, 3, 4):
         say(f"case C{which} sfxt method {method}")
         g.reset()
         g.report_paths(10, sfxt_method=method, kernel_timing=True)
@@ -388,12 +391,13 @@ def test_sfxt_dataflow_cycle_terminates():
     w = np.array([1, 2, 3, 4, 5], np.float32)
     say("case cycle graph")
     res = []
-    for method in (0, 1):
+    for method in (1, 2, 3, 4):
         g = engine_for(5, u, v, w)
         g.report_paths(10, sfxt_method=method)
         res.append(g.sfxt() + (g.get_slacks(10),))
-    for a, b in zip(res[0], res[1]):
-        np.testing.assert_array_equal(a, b)
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(res[0][2], [-1, -1, -1, 0, 1])
 
 

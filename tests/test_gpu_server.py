@@ -131,3 +131,47 @@ def test_full_admission_group_on_one_graph(nq):
         want, _ = og.topk(d, s, k)
         got = S.get_slacks(qid, k)
         assert len(got) == len(want) and pyoracle.first_bit_mismatch(want, got) == -1, qid
+
+
+def test_served_stream_graphs_equal_oracle():
+    """Queries on the served-stream graphs themselves (200K-2.6M vertices,
+    4M-70M edges, mean fanout 2.5-30): endpoint subsets of 1-100% of the
+    sinks and K from 1K to 100K, all in flight together; every result equals
+    the oracle's masked top-K bit for bit."""
+    gids = [0, 2, 3, 4, 5, 7]
+    graphs = {i: cpgamd.gen_preset(5, i) for i in gids}
+    S = cpgamd.Server(max_in_flight=64)
+    try:
+        slot = {}
+        for i in gids:
+            slot[i] = S.add_graph(*graphs[i])
+        rng = np.random.default_rng(555)
+        plan = [(7, 1.0, 100000), (7, 0.05, 20000)]
+        for _ in range(12):
+            gi = int(rng.choice(gids[:-1]))
+            frac = float(rng.choice([0.01, 0.1, 0.4, 0.8, 1.0]))
+            k = int(rng.choice([1000, 10000, 50000, 100000]))
+            plan.append((gi, frac, k))
+        sub = {}
+        for gi, frac, k in plan:
+            n, u, v, w = graphs[gi]
+            m = None if frac == 1.0 else sink_mask(rng, n, u, frac)
+            sub[S.submit(slot[gi], k, m)] = (gi, k, m, frac)
+        say(f"case served-stream graphs, {len(sub)} queries")
+        run_stream(S)
+        say(f"server stats {S.stats()}")
+        bad = []
+        for gi in gids:
+            og = pyoracle.OracleGraph.from_edges(*graphs[gi])
+            for qid, (g2, k, m, frac) in sub.items():
+                if g2 != gi:
+                    continue
+                d, s, _, _ = og.sfxt(m)
+                want, _ = og.topk(d, s, k)
+                got = S.get_slacks(qid, k)
+                if len(got) != len(want) or pyoracle.first_bit_mismatch(want, got) != -1:
+                    bad.append((gi, k, frac, len(got), len(want)))
+            del og
+        assert not bad, bad
+    finally:
+        S.close()

@@ -1,0 +1,110 @@
+"""The engine's unchanged HIP sources on the CPU SIMT emulator (tests/simt):
+blocks run as OS threads, waves as fibers, cooperative launches with every
+block resident.  These runs check on the CPU what the GPU suite checks on
+the device: both SFXT loops give the oracle's dist/succ/level, a query
+gives the oracle's costs bit for bit, a node-store overflow restarts
+exactly, and the dataflow loop's LDS pool spills to its global queue.
+SIMT_GUARD=1 places every device buffer against an inaccessible page, so
+an access past a buffer's end faults here instead of on a GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cpgamd
+import pyoracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SIMT = os.path.join(ROOT, "tests", "simt")
+LIB = os.path.join(SIMT, "libcpgamd_simt.so")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    """The emulator build of the C ABI, loaded in place of the HIP one for
+    this module's tests only."""
+    jobs = str(min(8, os.cpu_count() or 1))
+    subprocess.run(["make", "-s", "-j", jobs, "-C", SIMT], check=True)
+    os.environ.setdefault("SIMT_CUS", "4")
+    os.environ.setdefault("SIMT_GUARD", "1")
+    saved = cpgamd._lib
+    cpgamd._lib = cpgamd.load_library(LIB)
+    try:
+        yield cpgamd
+    finally:
+        cpgamd._lib = saved
+
+
+def bits_equal(want, got):
+    return len(want) == len(got) and pyoracle.first_bit_mismatch(want, got) == -1
+
+
+@pytest.fixture(scope="module")
+def c1():
+    n, u, v, w = cpgamd.gen_preset(1)
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, level, L = og.sfxt()
+    return n, u, v, w, og, dist, succ, level, L
+
+
+@pytest.mark.parametrize("method", [1, 2, 3, 4])
+def test_emulated_c1_query_equals_oracle(emu, c1, method):
+    n, u, v, w, og, dist, succ, level, L = c1
+    g = emu.CpGen()
+    try:
+        g.load_edges(n, u, v, w)
+        g.report_paths(100, sfxt_method=method)
+        gd, gs, gl = g.sfxt()
+        np.testing.assert_array_equal(gd, dist)
+        np.testing.assert_array_equal(gs, succ)
+        np.testing.assert_array_equal(gl, level)
+        assert g.graph_diameter == L
+        want, _ = og.topk(dist, succ, 100)
+        assert bits_equal(want, g.get_slacks(100))
+    finally:
+        g.close()
+
+
+def test_emulated_node_store_overflow_restarts_exactly(emu, c1):
+    n, u, v, w, og, dist, succ, level, L = c1
+    g = emu.CpGen()
+    try:
+        g.load_edges(n, u, v, w)
+        g.report_paths(2000, max_nodes=2500)
+        assert g.timings()["retries"] >= 1
+        want, _ = og.topk(dist, succ, 2000)
+        assert bits_equal(want, g.get_slacks(2000))
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("method", [2, 3])
+def test_emulated_lds_queue_spill(emu, method):
+    """One sink with 3000 fanins: settling it releases more vertices than a
+    block's LDS queue holds (the dataflow loop's pool, the local-queue
+    loop's level buffer), so both spill to their global queues."""
+    fan = 3000
+    rng = np.random.default_rng(11)
+    u, v = [], []
+    for i in range(fan):
+        u += [2 + i, 2 + fan + i, 1]
+        v += [0, 2 + i, 2 + i]
+    n = 2 + 2 * fan
+    u = np.array(u, np.int32)
+    v = np.array(v, np.int32)
+    w = rng.uniform(0.01, 1.0, len(u)).astype(np.float32)
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, level, L = og.sfxt()
+    g = emu.CpGen()
+    try:
+        g.load_edges(n, u, v, w)
+        g.report_paths(50, sfxt_blocks=2, sfxt_method=method)
+        gd, gs, gl = g.sfxt()
+        np.testing.assert_array_equal(gd, dist)
+        np.testing.assert_array_equal(gs, succ)
+        np.testing.assert_array_equal(gl, level)
+        want, _ = og.topk(dist, succ, 50)
+        assert bits_equal(want, g.get_slacks(50))
+    finally:
+        g.close()
