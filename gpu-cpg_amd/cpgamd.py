@@ -63,6 +63,17 @@ class ServerStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("group", ctypes.c_int), ("grid_mult", ctypes.c_int),
+                ("target_div", ctypes.c_int * 3), ("trials", ctypes.c_int),
+                ("start_ms", ctypes.c_double), ("best_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"group": self.group, "grid_mult": self.grid_mult,
+                "target_div": list(self.target_div), "trials": self.trials,
+                "start_ms": round(self.start_ms, 3), "best_ms": round(self.best_ms, 3)}
+
+
 class EdgeList(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("m", ctypes.c_int64),
                 ("u", ctypes.POINTER(ctypes.c_int)), ("v", ctypes.POINTER(ctypes.c_int)),
@@ -96,6 +107,8 @@ ABI = {
     "cpg_get_path_vertices": (_I64, [_H, _I64, _PI, _I64]),
     "cpg_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "cpg_step_times": (_I64, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64), _I64]),
+    "cpg_set_csr_reorder": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cpg_reorder_ms": (ctypes.c_double, [_H]),
     "cpg_sfxt": (ctypes.c_int, [_H]),
     "cpg_get_sfxt": (ctypes.c_int, [_H, _PI, _PI, _PI]),
     "cpg_batch_create": (_H, []),
@@ -103,6 +116,9 @@ ABI = {
     "cpg_batch_add_edges": (ctypes.c_int, [_H, ctypes.c_int, _I64, _PI, _PI, _PF, _I64]),
     "cpg_batch_num_queries": (ctypes.c_int, [_H]),
     "cpg_batch_run": (ctypes.c_int, [_H, ctypes.POINTER(QueryOpts)]),
+    "cpg_batch_autotune": (ctypes.c_int, [_H, ctypes.POINTER(QueryOpts), ctypes.c_int]),
+    "cpg_batch_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(BatchTuning)]),
+    "cpg_batch_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(BatchTuning)]),
     "cpg_batch_get_costs": (_I64, [_H, ctypes.c_int, _I64, _PF]),
     "cpg_batch_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
     "cpg_server_create": (_H, [_I64, ctypes.c_int, _I64]),
@@ -371,6 +387,14 @@ class CpGen:
     def run_sfxt(self):
         _c(self, self.L.cpg_sfxt(self.h))
 
+    def set_csr_reorder(self, method):
+        """CsrReorderMethod of the resident graph: 0 NONE, 1 E_ORIENTED
+        (level-order relabelling on the device; results unchanged)."""
+        _c(self, self.L.cpg_set_csr_reorder(self.h, int(method)))
+
+    def reorder_ms(self):
+        return float(self.L.cpg_reorder_ms(self.h))
+
     # -- BVSS deviation discovery (gpucpg/tc_pfxt_bvss.cuh)
     def bvss_build(self):
         _c(self, self.L.cpg_bvss_build(self.h))
@@ -463,6 +487,27 @@ class Batch:
         w = np.ascontiguousarray(w, np.float32)
         return _check(self.L.cpg_batch_add_edges(self.h, int(n), len(u), _iptr(u), _iptr(v),
                                                  _fptr(w), int(k)), self.L)
+
+    def autotune(self, max_trials=16):
+        """Chooses the round shapes by timing whole batch runs
+        (cpg_batch_autotune); returns the chosen shapes."""
+        o = QueryOpts()
+        self.L.cpg_query_opts_default(ctypes.byref(o))
+        _check(self.L.cpg_batch_autotune(self.h, ctypes.byref(o), int(max_trials)), self.L)
+        return self.tuning()
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(self.L.cpg_batch_get_tuning(self.h, ctypes.byref(t)), self.L)
+        return t.as_dict()
+
+    def set_tuning(self, group=4, grid_mult=8, target_div=(8, 8, 8)):
+        t = BatchTuning()
+        t.group = int(group)
+        t.grid_mult = int(grid_mult)
+        for i in range(3):
+            t.target_div[i] = int(target_div[i])
+        _check(self.L.cpg_batch_set_tuning(self.h, ctypes.byref(t)), self.L)
 
     def run(self, root_tie_quirk=True, init_split_perc=0.005, sfxt_blocks=0, sfxt_method=0):
         o = QueryOpts()

@@ -175,6 +175,18 @@ struct cpg_handle {
   int cache_hits = 0, cache_misses = 0;
   std::vector<int> h_succ;  // lazily copied for path reconstruction
 
+  // CSR relabelling (cpg_set_csr_reorder; 0 none, 1 level order as the
+  // reference's E_ORIENTED reorder): the device graph is held relabelled,
+  // h_perm maps original -> relabelled ids and h_orig back.  Every result
+  // leaves the engine in original ids, and ties break on original ids, so
+  // results equal those of the unrelabelled graph.
+  int csr_reorder = 0;
+  bool relabelled = false;
+  std::vector<int> h_perm, h_orig;
+  DevBuf<int> perm_d, orig_d;
+  DevBuf<uint8_t> mask_tmp;
+  double reorder_ms = 0.0;
+
   // PFXT
   DevBuf<float> n_cost;
   DevBuf<int> n_to, n_from, n_parent, n_level;
@@ -262,6 +274,7 @@ struct cpg_handle {
     g = std::move(ng);
     loaded = true;
     graph_on_device = false;
+    relabelled = false;
     last_diameter = 0;
     invalidate_cache_payload();
     cache_hits = cache_misses = 0;  // read_input clears the static cache (gpucpg.cu:454)
@@ -309,7 +322,74 @@ struct cpg_handle {
     CPG_HIP(hipStreamSynchronize(stream));
     graph_on_device = true;
     sfxt_valid = false;
+    relabelled = false;
+    if (csr_reorder == 1) relabel_graph();
   }
+
+  // Level-order relabelling of the device graph (E_ORIENTED): levels from
+  // one SFXT of the graph as loaded, the permutation by a counting sort of
+  // the levels (ties and unlevelled vertices in ascending id), then the
+  // edge-parallel scatter of both CSRs on the device.
+  void relabel_graph() {
+    const double t0 = now_ms();
+    relabelled = false;
+    const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
+    if (n == 0) return;
+    run_sfxt(false, nullptr);
+    std::vector<int> lv(n);
+    d2h(lv.data(), level.p, n, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    const int L = std::max(diameter, 0);
+    std::vector<int> start(static_cast<size_t>(L) + 2, 0);
+    for (size_t v = 0; v < n; ++v) start[static_cast<size_t>(lv[v] < 0 || lv[v] >= L ? L : lv[v]) + 1]++;
+    for (int b = 0; b <= L; ++b) start[b + 1] += start[b];
+    h_orig.assign(n, 0);
+    h_perm.assign(n, 0);
+    for (size_t v = 0; v < n; ++v) {
+      const int b = lv[v] < 0 || lv[v] >= L ? L : lv[v];
+      const int p = start[b]++;
+      h_orig[p] = static_cast<int>(v);
+      h_perm[v] = p;
+    }
+    std::vector<int> noadjp(n + 1, 0), niadjp(n + 1, 0);
+    for (size_t p = 0; p < n; ++p) {
+      const int v = h_orig[p];
+      noadjp[p + 1] = noadjp[p] + (g.oadjp[v + 1] - g.oadjp[v]);
+      niadjp[p + 1] = niadjp[p] + (g.iadjp[v + 1] - g.iadjp[v]);
+    }
+    perm_d.reserve(n); orig_d.reserve(n);
+    h2d(perm_d.p, h_perm.data(), n, stream);
+    h2d(orig_d.p, h_orig.data(), n, stream);
+    DevBuf<int> owner, nadjp, nadj, niadjp_d;
+    DevBuf<float> nw;
+    DevBuf<int2> nie;
+    owner.reserve(std::max<size_t>(m, 1));
+    nadjp.reserve(n + 1); niadjp_d.reserve(n + 1);
+    nadj.reserve(std::max<size_t>(m, 1)); nw.reserve(std::max<size_t>(m, 1)); nie.reserve(std::max<size_t>(m, 1));
+    h2d(nadjp.p, noadjp.data(), n + 1, stream);
+    h2d(niadjp_d.p, niadjp.data(), n + 1, stream);
+    launch_edge_owner(oadjp.p, g.n, owner.p, stream);
+    launch_reorder_fanout(oadjp.p, oadj.p, owgt.p, owner.p, perm_d.p, nadjp.p, static_cast<int64_t>(m),
+                          nadj.p, nw.p, stream);
+    launch_edge_owner(iadjp.p, g.n, owner.p, stream);
+    launch_reorder_fanin(iadjp.p, iedge.p, owner.p, perm_d.p, niadjp_d.p, static_cast<int64_t>(m), nie.p,
+                         stream);
+    std::vector<int> ns(g.srcs.size()), nk(g.sinks.size());
+    for (size_t i = 0; i < ns.size(); ++i) ns[i] = h_perm[static_cast<size_t>(g.srcs[i])];
+    for (size_t i = 0; i < nk.size(); ++i) nk[i] = h_perm[static_cast<size_t>(g.sinks[i])];
+    h2d(srcs.p, ns.data(), ns.size(), stream);
+    h2d(sinks.p, nk.data(), nk.size(), stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    oadjp.swap(nadjp); oadj.swap(nadj); owgt.swap(nw); iadjp.swap(niadjp_d); iedge.swap(nie);
+    relabelled = true;
+    sfxt_valid = false;
+    h_succ.clear();
+    hb.valid = false;
+    reorder_ms = now_ms() - t0;
+  }
+
+  // Original id of a relabelled vertex id (-1 stays -1).
+  int to_orig(int v) const { return relabelled && v >= 0 ? h_orig[static_cast<size_t>(v)] : v; }
 
   // Level-loop grid: about one thread per vertex of an average level (known
   // from the last levelization of this graph), within [16, 64] blocks; a
@@ -360,6 +440,7 @@ struct cpg_handle {
     if (n > 0) {
       SfxtArgs a{};
       a.n = g.n;
+      a.m = g.m;
       a.nsinks = static_cast<int>(g.sinks.size());
       a.oadjp = oadjp.p; a.oadj = oadj.p; a.owgt = owgt.p;
       a.iadjp = iadjp.p; a.iedge = iedge.p; a.sinks = sinks.p; a.best = best.p;
@@ -370,15 +451,24 @@ struct cpg_handle {
       a.ctl = sctl.p;
       if (!subset.empty()) {
         emask.reserve(n);
-        h2d(emask.p, mask, n, stream);
+        if (relabelled) {  // the mask is indexed by original id
+          mask_tmp.reserve(n);
+          h2d(mask_tmp.p, mask, n, stream);
+          launch_gather_u8(mask_tmp.p, orig_d.p, g.n, emask.p, stream);
+        } else {
+          h2d(emask.p, mask, n, stream);
+        }
         a.sink_mask = emask.p;
       }
+      a.orig = relabelled ? orig_d.p : nullptr;
       // 0 (automatic) is the fence-free level loop with its working blocks
       // on one XCD (round-robin dispatch puts every 8th block on the same
       // XCD): 3.67 ms on the 1.5M-vertex DAG against 4.2 ms for the fenced
       // loop over all XCDs
       const int method = sfxt_method == 0 ? 4 : sfxt_method;
       const bool dataflow = method == 2;
+      if (dataflow && relabelled)
+        throw std::runtime_error("the dataflow SFXT loop does not run on a relabelled graph");
       a.place_stride = sfxt_method == 0 ? kXcds : 1;
       int df_grid = 0;
       if (dataflow) {
@@ -401,6 +491,7 @@ struct cpg_handle {
                            : method == 4 ? launch_sfxt_levels_nf(a, level_loop_blocks(), stream)
                                          : launch_sfxt_levels(a, level_loop_blocks(), stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.b, stream));
+      if (relabelled) launch_sfxt_succ_relabel(succ.p, perm_d.p, g.n, stream);
       d2h(&c, sctl.p, 1, stream);
       CPG_HIP(hipStreamSynchronize(stream));
       if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
@@ -822,6 +913,11 @@ struct cpg_handle {
       d2h(h_to.data(), r_to.p, n, stream);
       d2h(h_parent.data(), r_parent.p, n, stream);
       CPG_HIP(hipStreamSynchronize(stream));
+      if (relabelled)
+        for (size_t i = 0; i < n; ++i) {
+          h_from[i] = to_orig(h_from[i]);
+          h_to[i] = to_orig(h_to[i]);
+        }
     }
     host_result = true;
   }
@@ -872,12 +968,17 @@ struct cpg_handle {
       if (path.size() > static_cast<size_t>(g.n) + 1) throw std::runtime_error("path does not end");
     }
     if (di != 0) throw std::runtime_error("deviation vertex not on the suffix chain");
+    if (relabelled)
+      for (int& v2 : path) v2 = to_orig(v2);
     return path;
   }
 
   // A_dev index: one entry per (64-source interval, destination row) with
   // a deviation edge, entries ordered by interval then row.
   void bvss_build() {
+    if (csr_reorder != 0)
+      throw std::runtime_error("the BVSS index is built on the graph in its original vertex order "
+                               "(csr reorder off)");
     if (!sfxt_valid) run_sfxt(false);
     fetch_succ();
     const int n = g.n;
@@ -1025,6 +1126,21 @@ struct cpg_batch {
   hipGraphExec_t graph_exec = nullptr;
   MqArgs graph_args{};   // the arguments the captured round was recorded with
   int graph_grid = 0;
+  // Round shapes (cpg_batch_autotune picks them by measurement):
+  //   group       replayed rounds per host synchronisation
+  //   grid_mult   round kernels' grid = CUs x grid_mult blocks
+  //   target_div  per K class (K < 1000, < 10000, >= 10000): a query
+  //               expands about K / div parents per round
+  cpg_batch_tuning tune = default_tuning();
+
+  static cpg_batch_tuning default_tuning() {
+    cpg_batch_tuning t{};
+    t.group = 4;
+    t.grid_mult = 8;
+    t.target_div[0] = t.target_div[1] = t.target_div[2] = 8;
+    return t;
+  }
+  static int k_class(uint64_t k) { return k < 1000 ? 0 : k < 10000 ? 1 : 2; }
 
   ~cpg_batch() { drop_graph(); }
 
@@ -1063,6 +1179,51 @@ struct cpg_batch {
     qs.reserve(nqc); mctl.reserve(1);
     hist.reserve(static_cast<size_t>(nqc) * kMqHistStride);
     cap = c;
+  }
+
+  // Measurement-driven choice of the round shapes: coordinate descent
+  // from the current shapes, one whole batch run (SFXT, rounds, selection)
+  // per trial, a candidate kept when it is at least 2% faster.  Results do
+  // not depend on the shapes (every shape gives the exact top K).
+  void autotune(const cpg_query_opts& o, int max_trials) {
+    if (max_trials < 1) throw std::runtime_error("max_trials must be >= 1");
+    auto timed = [&]() {
+      const double t0 = now_ms();
+      run(o);
+      return now_ms() - t0;
+    };
+    timed();  // sizes the buffers and captures the round
+    cpg_batch_tuning best = tune;
+    double best_ms = timed();
+    const double start_ms = best_ms;
+    int trials = 1;
+    static const int kGroups[] = {2, 4, 8, 16};
+    static const int kGrids[] = {4, 8, 16};
+    static const int kDivs[] = {2, 4, 8, 16, 32};
+    auto try_value = [&](int* field, const int* vals, int nv) {
+      const int keep = *field;
+      int pick = keep;
+      for (int i = 0; i < nv && trials < max_trials; ++i) {
+        if (vals[i] == keep) continue;
+        *field = vals[i];
+        ++trials;
+        const double ms = timed();
+        if (ms < best_ms * 0.98) {
+          best_ms = ms;
+          pick = vals[i];
+        }
+      }
+      *field = pick;
+      best = tune;
+    };
+    try_value(&tune.group, kGroups, 4);
+    try_value(&tune.grid_mult, kGrids, 3);
+    for (int c = 2; c >= 0; --c) try_value(&tune.target_div[c], kDivs, 5);
+    tune = best;
+    tune.trials = trials;
+    tune.best_ms = best_ms;
+    tune.start_ms = start_ms;
+    run(o);  // leave the results of the chosen shapes
   }
 
   void run(const cpg_query_opts& o) {
@@ -1131,7 +1292,8 @@ struct cpg_batch {
       m.klo = cnt ? float_key(minc) : 0u;
       m.shift = window_shift_bins(m.klo, kKeyInf, kMqBins);
       m.k = static_cast<unsigned long long>(std::max<int64_t>(0, designs[q].k));
-      m.batch_target = std::max<unsigned long long>(256, m.k / 8);
+      m.batch_target = std::max<unsigned long long>(
+          256, m.k / static_cast<unsigned long long>(std::max(1, tune.target_div[k_class(m.k)])));
       m.done = (cnt == 0 || m.k == 0) ? 1 : 0;
       m.vrec = uh.vrec.p;
       m.devinfo = uh.devinfo.p;
@@ -1159,22 +1321,23 @@ struct cpg_batch {
         if (r > 100000) throw std::runtime_error("batched PFXT round limit exceeded");
         if (now_ms() - t0 > CPG_QUERY_LIMIT_MS) throw std::runtime_error("batched query exceeded its time limit");
         // round 0 expands every root; later rounds replay the captured round
-        const int group = r == 0 ? 1 : 4;
+        const int group = r == 0 ? 1 : std::max(1, tune.group);
+        const int rgrid = std::max(1, uh.grid / 8 * std::max(1, tune.grid_mult));
         if (r == 0) {
-          launch_mq_round(a, true, uh.grid, stream);
+          launch_mq_round(a, true, rgrid, stream);
         } else {
           // the captured round holds buffer addresses and sizes: capture
           // again whenever any of them changed (node store growth, designs
-          // added, a rebuilt union graph)
-          if (graph_exec && (std::memcmp(&graph_args, &a, sizeof(MqArgs)) != 0 || graph_grid != uh.grid))
+          // added, a rebuilt union graph, another grid)
+          if (graph_exec && (std::memcmp(&graph_args, &a, sizeof(MqArgs)) != 0 || graph_grid != rgrid))
             drop_graph();
           if (!graph_exec) {
             CPG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            launch_mq_round(a, false, uh.grid, stream);
+            launch_mq_round(a, false, rgrid, stream);
             CPG_HIP(hipStreamEndCapture(stream, &graph));
             CPG_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
             graph_args = a;
-            graph_grid = uh.grid;
+            graph_grid = rgrid;
           }
           for (int gi = 0; gi < group; ++gi) CPG_HIP(hipGraphLaunch(graph_exec, stream));
         }
@@ -1278,8 +1441,11 @@ struct cpg_server {
     std::unique_ptr<cpg_handle> h;
     std::vector<std::unique_ptr<IndexSet>> pool;  // index buffers of finished queries
     int64_t index_bytes = 0;
+    // stored paths per asked path of this graph's finished queries (an
+    // exponential average; admission's node estimate)
+    double nodes_per_k = static_cast<double>(kNodesPerK);
     struct Multi {  // the multi-subset SFXT of an admission group
-      DevBuf<unsigned long long> best;
+      DevBuf<int> dv;  // settled distances, vertex-major (n x Q)
       DevBuf<int> dist, succ;
       DevBuf<float> distf;
       DevBuf<uint8_t> masks;
@@ -1309,6 +1475,7 @@ struct cpg_server {
     std::unique_ptr<IndexSet> index;
     std::vector<int> roots;     // kept roots (vertex ids, ascending)
     uint64_t seed_n = 0;        // paths of the first-bound probe left in the graph handle's store
+    int64_t node_est = 0;       // node-store estimate held while admitted (0 = not held)
     MqQuery init{};             // slot state at admission
     std::vector<float> result;
   };
@@ -1336,6 +1503,20 @@ struct cpg_server {
   int64_t next_id = 1;
   int64_t budget = 0;
   int64_t index_bytes = 0;          // indexes of the in-flight queries
+  // Node-store admission: every round scans the whole store (histograms,
+  // batch selection), so its cost grows with the nodes the in-flight
+  // queries hold.  A query is expected to store about kNodesPerK x K
+  // paths (7.04 x K on the 1.5M-vertex DAG at K = 1M); admission keeps the
+  // in-flight estimate within node_budget, a lone query always admitted.
+  static constexpr int64_t kNodesPerK = 8;
+  int64_t node_budget = int64_t(128) << 20;
+  int64_t nodes_held = 0;           // node estimates of the admitted queries
+  // a query's node estimate, fixed when it is admitted (released with it)
+  int64_t node_estimate(const Query& q) const {
+    if (q.node_est > 0) return q.node_est;
+    const double r = graphs[q.graph]->nodes_per_k;
+    return static_cast<int64_t>(r * static_cast<double>(std::max<int64_t>(q.k, 0))) + 4096;
+  }
   int nslots = 0;
   int in_flight = 0;
   hipStream_t stream = nullptr;
@@ -1475,7 +1656,13 @@ struct cpg_server {
         const float c = static_cast<float>(sd[i]) / static_cast<float>(kScaleUp);
         if (c != 0.0f) rc.push_back(c);
       }
-    const bool roots_bound = q.k > 0 && static_cast<int64_t>(rc.size()) >= q.k;
+    // The K-th root cost bounds the answer, but loosely: the children of
+    // the cheapest roots are cheaper than the K-th root, and a first round
+    // under that bound stores every child up to it.  It is taken only
+    // where the roots' chains are short (the single query's rule for
+    // skipping its probe); otherwise the probe finds a tight bound.
+    const bool short_chains = nkept * (static_cast<int64_t>(h.diameter) + 1) <= (int64_t(1) << 20);
+    const bool roots_bound = q.k > 0 && static_cast<int64_t>(rc.size()) >= q.k && short_chains;
     if (roots_bound) {
       std::nth_element(rc.begin(), rc.begin() + (q.k - 1), rc.end());
       const uint64_t cap = std::max<uint64_t>(uint64_t(1) << 16, static_cast<uint64_t>(nkept));
@@ -1596,12 +1783,14 @@ struct cpg_server {
         slot_reserved[q.slot] = 1;
         q.state = 3;
         index_bytes += G.index_bytes;
+        q.node_est = node_estimate(q);
+        nodes_held += q.node_est;
       }
       h.deps.reserve(n); h.queue.reserve(n); h.lvlp.reserve(n + 2); h.level.reserve(n);
       h.big.reserve(n); h.sctl.reserve(1);
       if (h.g.max_out_degree > kSfxtLdsSort) h.devs_tmp.reserve(m);
       Graph::Multi& M = G.mm;
-      M.best.reserve(n * Q); M.dist.reserve(n * Q); M.succ.reserve(n * Q); M.distf.reserve(n * Q);
+      M.dv.reserve(n * Q); M.dist.reserve(n * Q); M.succ.reserve(n * Q); M.distf.reserve(n * Q);
       const size_t ns = h.g.sinks.size();
       M.masks.reserve(std::max<size_t>(1, ns * Q)); M.maxd.reserve(Q); M.ctls.reserve(Q);
       // one flag per sink (in sink order) and subset; padding subsets choose none
@@ -1625,7 +1814,7 @@ struct cpg_server {
       a.ctl = h.sctl.p;
       SfxtMulti mm{};
       mm.q = Q;
-      mm.masks = M.masks.p; mm.best = M.best.p; mm.dist = M.dist.p; mm.succ = M.succ.p;
+      mm.masks = M.masks.p; mm.dv = M.dv.p; mm.dist = M.dist.p; mm.succ = M.succ.p;
       mm.distf = M.distf.p; mm.maxd = M.maxd.p;
       const int lv = sfxt_lift_levels(h.diameter);
       h.lift.reserve(static_cast<size_t>(lv) * std::max<size_t>(n, 1));
@@ -1816,6 +2005,8 @@ struct cpg_server {
       q.state = 0;
       if (q.index) G.pool.push_back(std::move(q.index));
       index_bytes -= G.index_bytes;
+      nodes_held -= q.node_est;
+      q.node_est = 0;
       queue.push_front(q.id);
     }
     G.ids.clear();
@@ -1869,6 +2060,7 @@ struct cpg_server {
     std::vector<uint8_t> in_batch(graphs.size(), 0);
     int held = 0;             // slots promised to the groups formed so far
     int64_t held_bytes = 0;
+    int64_t held_nodes = 0;
     while (!queue.empty() && static_cast<int>(batch.size()) < std::min(kMaxParts, parts_cap)) {
       if (static_cast<int>(free_slots.size()) - held <= 0) {
         if (dead_slots.empty()) break;
@@ -1889,17 +2081,25 @@ struct cpg_server {
       const int avail = static_cast<int>(free_slots.size()) - held;
       std::vector<int64_t> ids;
       int64_t bytes = index_bytes + held_bytes;
+      int64_t nodes = nodes_held + held_nodes;
       for (int64_t id : queue) {
         if (static_cast<int>(ids.size()) >= std::min(16, avail)) break;
         if (queries[id]->graph != gi) continue;
-        if ((!idle || !ids.empty()) && bytes + gb > budget) break;  // HBM admission
+        const int64_t ne = node_estimate(*queries[id]);
+        // the store as it stands (live and not yet compacted) counts too
+        const int64_t store = static_cast<int64_t>(std::min<uint64_t>(node_n, st[cur].cap));
+        if ((!idle || !ids.empty()) &&
+            (bytes + gb > budget || nodes + ne > node_budget || store > node_budget))
+          break;
         ids.push_back(id);
         bytes += gb;
+        nodes += ne;
       }
       if (ids.empty()) break;
       for (int64_t id : ids) queue.erase(std::find(queue.begin(), queue.end(), id));
       held += static_cast<int>(ids.size());
       held_bytes += gb * static_cast<int64_t>(ids.size());
+      for (int64_t id : ids) held_nodes += node_estimate(*queries[id]);
       in_batch[gi] = 1;
       batch.emplace_back(gi, std::move(ids));
     }
@@ -1977,6 +2177,12 @@ struct cpg_server {
       if (!hs[sl]) continue;
       Query& q = *queries[slot_query[sl]];
       const uint64_t take = std::min<uint64_t>(pq[sl], static_cast<uint64_t>(q.k));
+      {
+        Graph& Gq = *graphs[q.graph];
+        const double used = static_cast<double>(hq[sl].stored + q.seed_n + q.roots.size());
+        const double r = used / static_cast<double>(std::max<int64_t>(q.k, 1));
+        Gq.nodes_per_k = 0.5 * Gq.nodes_per_k + 0.5 * std::max(1.0, r);
+      }
       q.result.assign(all.begin() + off, all.begin() + off + take);
       off += pq[sl];
       q.state = 2;
@@ -1985,6 +2191,8 @@ struct cpg_server {
       Graph& G = *graphs[q.graph];
       G.pool.push_back(std::move(q.index));
       index_bytes -= G.index_bytes;
+      nodes_held -= q.node_est;
+      q.node_est = 0;
       --in_flight;
       slot_query[sl] = -1;
       dead_slots.push_back(sl);
@@ -2004,6 +2212,8 @@ struct cpg_server {
       q.slot = -1;
       back.push_back(q.id);
       index_bytes -= graphs[q.graph]->index_bytes;
+      nodes_held -= q.node_est;
+      q.node_est = 0;
       graphs[q.graph]->pool.push_back(std::move(q.index));  // prepared again on re-admission
       slot_query[sl] = -1;
     }
@@ -2212,9 +2422,16 @@ int64_t cpg_num_paths(cpg_handle* h) { return h && h->have_result ? h->nres : -1
 int64_t cpg_get_costs(cpg_handle* h, int64_t k, float* out) {
   CPG_CHECK_HANDLE(h, -1)
   return guarded<int64_t>(-1, [&]() {
-    h->fetch_result();
+    if (!h->have_result) throw std::runtime_error("no query result");
     const int64_t n = std::min<int64_t>(std::max<int64_t>(k, 0), h->nres);
-    if (n && out) std::memcpy(out, h->h_cost.data(), sizeof(float) * n);
+    if (n && out) {
+      if (h->host_result) {
+        std::memcpy(out, h->h_cost.data(), sizeof(float) * n);
+      } else {  // only the costs cross PCIe (the other report arrays stay on the device)
+        d2h(out, h->r_cost.p, static_cast<size_t>(n), h->stream);
+        CPG_HIP(hipStreamSynchronize(h->stream));
+      }
+    }
     return n;
   });
 }
@@ -2264,6 +2481,22 @@ int64_t cpg_step_times(cpg_handle* h, double* ms, int64_t* paths, int64_t cap) {
   return n;
 }
 
+int cpg_set_csr_reorder(cpg_handle* h, int method) {
+  CPG_CHECK_HANDLE(h, -1)
+  return guarded<int>(-1, [&]() {
+    if (method < 0 || method > 1) throw std::runtime_error("csr reorder method must be 0 (none) or 1 (level order)");
+    if (method != h->csr_reorder) {
+      h->csr_reorder = method;
+      h->graph_on_device = false;  // re-uploaded (and relabelled) on next use
+      h->relabelled = false;
+      h->invalidate_cache_payload();
+    }
+    return 0;
+  });
+}
+
+double cpg_reorder_ms(cpg_handle* h) { return h ? h->reorder_ms : -1.0; }
+
 int cpg_sfxt(cpg_handle* h) {
   CPG_CHECK_HANDLE(h, -1)
   return guarded<int>(-1, [&]() {
@@ -2279,10 +2512,26 @@ int cpg_get_sfxt(cpg_handle* h, int* dist_out, int* succ_out, int* level_out) {
     if (!h->loaded) throw std::runtime_error("no graph loaded");
     if (!h->sfxt_valid) h->run_sfxt(false);
     const size_t n = static_cast<size_t>(h->g.n);
-    if (dist_out) d2h(dist_out, h->dist.p, n, h->stream);
-    if (succ_out) d2h(succ_out, h->succ.p, n, h->stream);
-    if (level_out) d2h(level_out, h->level.p, n, h->stream);
-    CPG_HIP(hipStreamSynchronize(h->stream));
+    if (!h->relabelled) {
+      if (dist_out) d2h(dist_out, h->dist.p, n, h->stream);
+      if (succ_out) d2h(succ_out, h->succ.p, n, h->stream);
+      if (level_out) d2h(level_out, h->level.p, n, h->stream);
+      CPG_HIP(hipStreamSynchronize(h->stream));
+      return 0;
+    }
+    // relabelled graph: entry v of each array is original vertex v
+    std::vector<int> t(n);
+    int* outs[3] = {dist_out, succ_out, level_out};
+    int* srcs[3] = {h->dist.p, h->succ.p, h->level.p};
+    for (int k = 0; k < 3; ++k) {
+      if (!outs[k]) continue;
+      d2h(t.data(), srcs[k], n, h->stream);
+      CPG_HIP(hipStreamSynchronize(h->stream));
+      for (size_t v = 0; v < n; ++v) {
+        const int x = t[static_cast<size_t>(h->h_perm[v])];
+        outs[k][v] = k == 1 ? h->to_orig(x) : x;
+      }
+    }
     return 0;
   });
 }
@@ -2366,6 +2615,39 @@ int cpg_batch_add_edges(cpg_batch* b, int n, int64_t m, const int* eu, const int
 }
 
 int cpg_batch_num_queries(cpg_batch* b) { return b ? static_cast<int>(b->designs.size()) : -1; }
+
+int cpg_batch_autotune(cpg_batch* b, const cpg_query_opts* o, int max_trials) {
+  if (!b) { g_last_error = "null batch"; return -1; }
+  return guarded<int>(-1, [&]() {
+    cpg_query_opts d;
+    cpg_query_opts_default(&d);
+    if (o) {
+      if (o->struct_size != static_cast<int>(sizeof(cpg_query_opts)))
+        throw std::runtime_error("cpg_query_opts.struct_size mismatch");
+      d = *o;
+    }
+    if (d.endpoint_mask) throw std::runtime_error("endpoint_mask is not supported in a batch");
+    b->autotune(d, max_trials);
+    return b->tune.trials;
+  });
+}
+
+int cpg_batch_get_tuning(cpg_batch* b, cpg_batch_tuning* out) {
+  if (!b || !out) { g_last_error = "null argument"; return -1; }
+  *out = b->tune;
+  return 0;
+}
+
+int cpg_batch_set_tuning(cpg_batch* b, const cpg_batch_tuning* t) {
+  if (!b || !t) { g_last_error = "null argument"; return -1; }
+  if (t->group < 1 || t->group > 64 || t->grid_mult < 1 || t->grid_mult > 64 || t->target_div[0] < 1 ||
+      t->target_div[1] < 1 || t->target_div[2] < 1) {
+    g_last_error = "tuning values out of range";
+    return -1;
+  }
+  b->tune = *t;
+  return 0;
+}
 
 int cpg_batch_run(cpg_batch* b, const cpg_query_opts* o) {
   if (!b) { g_last_error = "null batch"; return -1; }

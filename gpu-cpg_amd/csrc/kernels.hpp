@@ -32,6 +32,9 @@ struct SfxtArgs {
   const int* iadjp;
   const int2* iedge;  // fanin records {x, w_e bits}, in-CSR order
   const int* sinks;   // out-degree-0 vertices: level 0
+  // Relabelled graph (level-order CSR): original id of every vertex, the
+  // successor tie-break's order (null: ids are the original ones)
+  const int* orig;
   const uint8_t* sink_mask;  // optional endpoint subset (n bytes, nonzero = chosen sink)
   unsigned long long* best;  // packed relax candidate per vertex (see sfxt.hip)
   int* deps;
@@ -54,6 +57,8 @@ struct SfxtArgs {
   // part (the others return at once); with a stride of 8 (the XCD count)
   // every working block lands on the same XCD under round-robin dispatch
   int place_stride;
+  int pad;
+  int64_t m;  // edges (the fence-free loop picks its records per step by m / n)
 };
 
 // Q endpoint subsets relaxed in one level loop (served queries): the
@@ -65,7 +70,7 @@ struct SfxtMulti {
   int q;
   int pad;
   const uint8_t* masks;       // Q x nsinks bytes, nonzero = chosen sink (null = all)
-  unsigned long long* best;   // n x Q
+  int* dv;                    // n x Q settled distances, vertex-major (the pull reads)
   int* dist;                  // Q x n
   int* succ;                  // Q x n
   float* distf;               // Q x n
@@ -109,6 +114,9 @@ int launch_sfxt_dataflow(const SfxtArgs& args, int blocks, hipStream_t stream);
 // barrier; same results as the level loop).  Returns the grid used.
 void launch_sfxt_local_init(const SfxtArgs& args, hipStream_t stream);
 int launch_sfxt_local(const SfxtArgs& args, int blocks, hipStream_t stream);
+// Relabelled graph: succ[] holds original ids after a level loop; map
+// them to the relabelled ids (perm: original -> relabelled).
+void launch_sfxt_succ_relabel(int* succ, const int* perm, int n, hipStream_t stream);
 // Deviation records, their sort, the lift tables and the nsv pointers.
 // m: the graph's edge count (chooses thread- or wave-per-vertex records).
 void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream);
@@ -212,6 +220,7 @@ struct MqQuery {
   int step_limit;                 // bound on one walk's steps
   int all;                        // the next round expands every unexpanded path
   int pad;
+  unsigned long long stored;      // children the fill passes stored for this query
 };
 
 struct MqCtl {
@@ -314,5 +323,19 @@ void launch_emit_report(const uint32_t* val, uint64_t nout, NodeStore nodes, flo
 void launch_link_parents(const int* opidx, const int* rank_of, uint64_t nout, int* oparent,
                          hipStream_t s);
 void launch_fill_int(int* p, int v, uint64_t n, hipStream_t s);
+
+// ---------------- CSR relabelling (reorder.hip) ----------------
+// E_ORIENTED (gpucpg/gpucpg.cu:2048-2079): one thread per edge copies edge
+// e of u to slot new_adjp[perm[u]] + (e - adjp[u]) of the relabelled CSR,
+// its head relabelled; each vertex keeps its edge order.  owner[e] = u is
+// filled first by launch_edge_owner.
+void launch_edge_owner(const int* adjp, int n, int* owner, hipStream_t stream);
+void launch_reorder_fanout(const int* adjp, const int* adj, const float* w, const int* owner,
+                           const int* perm, const int* new_adjp, int64_t m, int* new_adj,
+                           float* new_w, hipStream_t stream);
+void launch_reorder_fanin(const int* iadjp, const int2* iedge, const int* owner, const int* perm,
+                          const int* new_iadjp, int64_t m, int2* new_iedge, hipStream_t stream);
+// out[i] = in[idx[i]] for bytes (endpoint masks into relabelled order)
+void launch_gather_u8(const uint8_t* in, const int* idx, int n, uint8_t* out, hipStream_t stream);
 
 }  // namespace cpg

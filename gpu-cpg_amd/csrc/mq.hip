@@ -192,6 +192,7 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
     ChainWalk w;
     w.start(i < nb, c, to, U, qs.ukey != kKeyInf, qs.mag);
     unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
+    unsigned stored = 0;  // fill: children this lane stored for its parent
     while (__ballot(w.mode != 2)) {
       float cc = 0.0f;
       int cw = 0, from = 0;
@@ -219,9 +220,24 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
           a.nodes.key[slot] = float_key(cc);
           a.nodes.flag[slot] = 0;
           a.nq[slot] = q;
+          ++stored;
         } else {
           atomicExch(&ctl->overflow, 1);
         }
+      }
+    }
+    if (kFill) {
+      // per-query stored counts: one atomic per distinct query among the
+      // wave's lanes that stored children
+      uint64_t pend = __ballot(stored > 0);
+      while (pend) {
+        const int leader = __ffsll(static_cast<long long>(pend)) - 1;
+        const int lq = __shfl(q, leader);
+        const bool mine = stored > 0 && q == lq;
+        unsigned long long sum = mine ? stored : 0;
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (__lane_id() == leader) atomicAdd(&a.qs[lq].stored, sum);
+        pend &= ~__ballot(mine);
       }
     }
   }

@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) sfxt_init_kernel(SfxtArgs a) {
 // device-scope stores and read with device-scope loads (-1 until written),
 // best[] is read with device-scope loads, and the grid barrier has no
 // release/acquire fence; the queue must be initialised to the sinks and -1.
-template <bool kFenced>
+template <bool kFenced, int kU = kFaninUnroll>
 __device__ void levels_loop(const SfxtArgs& a) {
   __shared__ int s_q[kLdsQueue];
   __shared__ int s_qn;
@@ -282,29 +282,37 @@ __device__ void levels_loop(const SfxtArgs& a) {
       }
       const bool reach = du != kIntMax;
       const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
+      const int tie = a.orig ? a.orig[u] : u;  // successor tie-break: original id
       a.dist[u] = du;
       a.succ[u] = s;
       a.level[u] = d;
       a.distf[u] = df;
       if (reach) local_max = fmaxf(local_max, fabsf(df));
-      for (int j = b; j < e; j += kFaninUnroll) {
-        // kFaninUnroll fanin records; past the end the last one is loaded
-        // again (so every load is unconditional) and its atomics are skipped
-        int2 r[kFaninUnroll];
+      for (int j = b; j < e; j += kU) {
+        // kU fanin records; past the end the last one is loaded again, so
+        // every load is unconditional
+        int2 r[kU];
 #pragma unroll
-        for (int q = 0; q < kFaninUnroll; ++q) r[q] = a.iedge[min(j + q, e - 1)];
-        int old[kFaninUnroll];
+        for (int q = 0; q < kU; ++q) r[q] = a.iedge[min(j + q, e - 1)];
+        // Branch-free atomics, all in flight together: with a branch per
+        // record the compiler waits for each returning atomic before the
+        // next.  A slot past the end issues no-op atomics (min with the
+        // largest key, subtract 0), which change nothing wherever they
+        // land; they land on a vertex spread by thread and slot, since
+        // atomics on one address (the copied record) serialise.
+        int old[kU];
 #pragma unroll
-        for (int q = 0; q < kFaninUnroll; ++q) {
-          old[q] = 0;
-          if (j + q < e) {
-            const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
-            if (reach) atomicMin(&a.best[r[q].x], pack_cand(du + wi, u));
-            old[q] = atomicSub(&a.deps[r[q].x], 1);
-          }
+        for (int q = 0; q < kU; ++q) {
+          const bool in = j + q < e;
+          const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
+          const int x = in ? r[q].x
+                           : static_cast<int>((static_cast<unsigned>(gtid) * kU + static_cast<unsigned>(q)) %
+                                              static_cast<unsigned>(a.n));
+          atomicMin(&a.best[x], in && reach ? pack_cand(du + wi, tie) : ~0ull);
+          old[q] = atomicSub(&a.deps[x], in ? 1 : 0);
         }
 #pragma unroll
-        for (int q = 0; q < kFaninUnroll; ++q)
+        for (int q = 0; q < kU; ++q)
           if (j + q < e && old[q] == 1) push(r[q].x);
       }
     }
@@ -326,7 +334,12 @@ __device__ void levels_loop(const SfxtArgs& a) {
 }
 
 __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) { levels_loop<true>(a); }
-__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf_kernel(SfxtArgs a) { levels_loop<false>(a); }
+// The fence-free loop with 4 or 8 fanin records in flight per step, chosen
+// by the graph's mean in-degree (each slot past a vertex's last record
+// costs a no-op atomic pair; 16 per step measured slower on the 30M-edge
+// DAG: 11.5 ms against 10.4 ms).
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf_kernel(SfxtArgs a) { levels_loop<false, 8>(a); }
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf4_kernel(SfxtArgs a) { levels_loop<false, 4>(a); }
 
 // ---------------------------------------------------------------------------
 // Dataflow loop: the same levelize / relax / successor results as the level
@@ -814,6 +827,7 @@ __global__ void __launch_bounds__(kLocBlock) sfxt_local_kernel(SfxtArgs a) {
       }
       const bool reach = du != kIntMax;
       const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
+      const int tie = a.orig ? a.orig[u] : u;  // successor tie-break: original id
       a.dist[u] = du;
       a.succ[u] = s;
       a.level[u] = d;
@@ -832,12 +846,15 @@ __global__ void __launch_bounds__(kLocBlock) sfxt_local_kernel(SfxtArgs a) {
         for (int q = 0; q < kLocUnroll; ++q) {
           xb[q] = a.iadjp[r[q].x];
           xe[q] = a.iadjp[r[q].x + 1];
-          old[q] = 0;
-          if (j + q < e) {
-            const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
-            if (reach) atomicMin(&a.best[r[q].x], pack_cand(du + wi, u));
-            old[q] = atomicSub(&a.deps[r[q].x], 1);
-          }
+          // branch-free: a slot past the end issues no-op atomics on a
+          // vertex spread by thread and slot
+          const bool in = j + q < e;
+          const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
+          const int x = in ? r[q].x
+                           : static_cast<int>(((bid * kLocBlock + static_cast<unsigned>(t)) * kLocUnroll +
+                                               static_cast<unsigned>(q)) % static_cast<unsigned>(a.n));
+          atomicMin(&a.best[x], in && reach ? pack_cand(du + wi, tie) : ~0ull);
+          old[q] = atomicSub(&a.deps[x], in ? 1 : 0);
         }
 #pragma unroll
         for (int q = 0; q < kLocUnroll; ++q)
@@ -866,6 +883,15 @@ __global__ void __launch_bounds__(kLocBlock) sfxt_local_kernel(SfxtArgs a) {
   if (ok && bid == 0 && t == 0) ctl->num_levels = a.nsinks > 0 ? d + 1 : 0;
 }
 
+// Relabelled graph: the level loops store the successor's original id;
+// this maps it to the relabelled id before the deviation index is built.
+__global__ void __launch_bounds__(256) sfxt_succ_relabel_kernel(int* succ, const int* perm, int n) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const int s = succ[v];
+    if (s >= 0) succ[v] = perm[s];
+  }
+}
+
 // Per-vertex state before the multi-subset level loop.
 __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMulti m) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -875,7 +901,7 @@ __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMu
     a.deps[v] = a.oadjp[v + 1] - a.oadjp[v];
     a.level[v] = -1;
     for (int q = 0; q < m.q; ++q) {
-      m.best[static_cast<size_t>(v) * m.q + q] = ~0ull;
+      m.dv[static_cast<size_t>(v) * m.q + q] = kIntMax;
       m.dist[q * n + v] = kIntMax;
       m.succ[q * n + v] = -1;
       m.distf[q * n + v] = static_cast<float>(kIntMax) / static_cast<float>(kScaleUp);
@@ -887,12 +913,15 @@ __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMu
 }
 
 // The level loop for Q endpoint subsets at once.  Levels, the queue and
-// the fanin release are the single loop's; one wave takes a vertex: lanes
-// q < Q settle its distance in subset q from best[u*Q + q], then the wave
-// walks the fanin records 64/Q at a time with lane l on record l / Q and
-// subset l % Q, offering that subset's candidate to the record's fanin;
-// the subset-0 lane of each record releases it.  The per-level grid
-// barrier is shared by all Q subsets, so Q subsets cost about one loop.
+// the fanin release are the single loop's; one wave takes a vertex and
+// settles it in all Q subsets pull-style, as relax_bu_step and
+// update_successors_bu read the fanouts (gpucpg/gpucpg.cu:1167-1187,
+// :2606-2628): lane l reads fanout record l / Q in subset l % Q from the
+// vertex-major distances dv[w*Q + q] of the settled fanouts, and a
+// shuffle reduction over each subset's lanes gives the minimum and the
+// largest fanout id at it.  No per-subset atomics: the only atomics are
+// the fanin releases, shared by the subsets.  The per-level grid barrier
+// is shared by all Q subsets, so Q subsets cost about one loop.
 // It runs on the blocks [b0, b0 + nblk) of a launch that holds the level
 // loops of several graphs (one partition each, with its own barrier).
 __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigned bid, unsigned nblk) {
@@ -938,50 +967,80 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
     if (bid == 0 && threadIdx.x == 0) a.lvlp[d] = static_cast<int>(lb);
     for (unsigned i = lb + static_cast<unsigned>(wave); i < le; i += static_cast<unsigned>(nwaves)) {
       const int u = a.queue[i];
+      const int ob = a.oadjp[u], oe = a.oadjp[u + 1];
+      const int ib = a.iadjp[u], ie = a.iadjp[u + 1];
       int du = kIntMax, s = -1;
-      if (lane < Q) {
-        if (d == 0) {  // level 0 is the sink list in order: i is the sink's position
-          if (!m.masks || m.masks[static_cast<size_t>(lane) * a.nsinks + i]) du = 0;
-        } else {
-          const unsigned long long key = m.best[static_cast<size_t>(u) * Q + lane];
-          if (key != ~0ull) {
-            du = unpack_cand(key);
-            s = unpack_succ(key);
+      if (d == 0) {  // level 0 is the sink list in order: i is the sink's position
+        if (lane < Q && (!m.masks || m.masks[static_cast<size_t>(lane) * a.nsinks + i])) du = 0;
+      } else {
+        // pull: lane l reads fanout record l / Q (64/Q at a time,
+        // kMultiSteps steps in flight) in subset l % Q; every fanout sits on
+        // a lower level, so its distance in every subset is settled
+        int bd = kIntMax, bw = -1;
+        for (int j0 = ob; j0 < oe; j0 += kMultiSteps * R) {
+          int w[kMultiSteps], dw[kMultiSteps];
+          float wf[kMultiSteps];
+#pragma unroll
+          for (int t = 0; t < kMultiSteps; ++t) {
+            const int j = j0 + t * R + rec;
+            w[t] = j < oe ? a.oadj[j] : -1;
+            wf[t] = j < oe ? a.owgt[j] : 0.0f;
+          }
+#pragma unroll
+          for (int t = 0; t < kMultiSteps; ++t)
+            dw[t] = w[t] >= 0 ? m.dv[static_cast<size_t>(w[t]) * Q + sub] : kIntMax;
+#pragma unroll
+          for (int t = 0; t < kMultiSteps; ++t) {
+            if (dw[t] == kIntMax) continue;
+            const int c = dw[t] + static_cast<int>(wf[t] * static_cast<float>(kScaleUp));
+            if (c < bd || (c == bd && w[t] > bw)) {
+              bd = c;
+              bw = w[t];
+            }
           }
         }
+        // minimum over the lanes of one subset (lanes sub, sub + Q, ...),
+        // the largest fanout id among equal minima
+        for (int o = Q; o < 64; o <<= 1) {
+          const int od = __shfl_xor(bd, o);
+          const int ow = __shfl_xor(bw, o);
+          if (od < bd || (od == bd && ow > bw)) {
+            bd = od;
+            bw = ow;
+          }
+        }
+        du = bd;
+        s = bw;
+      }
+      if (lane < Q) {
         const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
         m.dist[lane * n + u] = du;
         m.succ[lane * n + u] = s;
         m.distf[lane * n + u] = df;
+        m.dv[static_cast<size_t>(u) * Q + lane] = du;
         if (du != kIntMax) local_max = fmaxf(local_max, fabsf(df));
       }
       if (lane == 0) a.level[u] = d;
-      const int my_du = __shfl(du, sub);
-      const int b = a.iadjp[u], e = a.iadjp[u + 1];
-      // kMultiSteps record steps per iteration: their loads and atomics are
-      // all in flight before the releases' old values are consumed
-      for (int j0 = b; j0 < e; j0 += kMultiSteps * R) {
-        int2 r[kMultiSteps];
+      // release: every lane takes fanin records (the subsets share the
+      // levels, so each record is released once)
+      for (int j0 = ib; j0 < ie; j0 += 64 * kMultiSteps) {
+        int x[kMultiSteps], old[kMultiSteps];
+        // branch-free, so the atomics are in flight together: a slot past
+        // the end subtracts 0 (a no-op) from a vertex spread by lane and
+        // step (atomics on one address serialise)
 #pragma unroll
         for (int t = 0; t < kMultiSteps; ++t) {
-          const int j = j0 + t * R + rec;
-          r[t] = j < e ? a.iedge[j] : make_int2(0, 0);
+          const int j = j0 + t * 64 + lane;
+          x[t] = j < ie ? a.iedge[j].x
+                        : static_cast<int>((static_cast<unsigned>(wave) * 64u * kMultiSteps +
+                                            static_cast<unsigned>(t * 64 + lane)) %
+                                           static_cast<unsigned>(a.n));
         }
-        int old[kMultiSteps];
 #pragma unroll
-        for (int t = 0; t < kMultiSteps; ++t) {
-          old[t] = 0;
-          if (j0 + t * R + rec < e) {
-            if (my_du != kIntMax) {
-              const int wi = static_cast<int>(__int_as_float(r[t].y) * static_cast<float>(kScaleUp));
-              atomicMin(&m.best[static_cast<size_t>(r[t].x) * Q + sub], pack_cand(my_du + wi, u));
-            }
-            if (sub == 0) old[t] = atomicSub(&a.deps[r[t].x], 1);
-          }
-        }
+        for (int t = 0; t < kMultiSteps; ++t) old[t] = atomicSub(&a.deps[x[t]], j0 + t * 64 + lane < ie ? 1 : 0);
 #pragma unroll
         for (int t = 0; t < kMultiSteps; ++t)
-          if (j0 + t * R + rec < e && sub == 0 && old[t] == 1) push(r[t].x);
+          if (j0 + t * 64 + lane < ie && old[t] == 1) push(x[t]);
       }
     }
     const unsigned pushed = flush();
@@ -1347,8 +1406,11 @@ int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
 
 int launch_sfxt_levels_nf(const SfxtArgs& args, int blocks, hipStream_t stream) {
   const int stride = std::max(1, args.place_stride);
-  const int grid = placed_grid(blocks, resident_blocks(sfxt_levels_nf_kernel, kSfxtBlock), stride);
-  launch_resident(sfxt_levels_nf_kernel, grid, kSfxtBlock, args, stream);
+  // mean in-degree m / n: under 4 -> 4 records per step, else 8
+  const int64_t m = args.m, n = std::max(1, args.n);
+  auto k = m < 4 * n ? sfxt_levels_nf4_kernel : sfxt_levels_nf_kernel;
+  const int grid = placed_grid(blocks, resident_blocks(k, kSfxtBlock), stride);
+  launch_resident(k, grid, kSfxtBlock, args, stream);
   return grid / stride;
 }
 
@@ -1380,6 +1442,13 @@ int launch_sfxt_dataflow(const SfxtArgs& args, int blocks, hipStream_t stream) {
   hipLaunchKernelGGL(sfxt_dataflow_kernel, dim3(grid), dim3(kDfBlock), 0, stream, args);
   CPG_HIP(hipGetLastError());
   return grid;
+}
+
+void launch_sfxt_succ_relabel(int* succ, const int* perm, int n, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(sfxt_succ_relabel_kernel, dim3(flat_grid(n, device_cus())), dim3(256), 0, stream,
+                     succ, perm, n);
+  CPG_HIP(hipGetLastError());
 }
 
 void launch_sfxt_local_init(const SfxtArgs& args, hipStream_t stream) {

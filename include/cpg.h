@@ -163,6 +163,16 @@ int64_t cpg_step_times(cpg_handle* h, double* ms, int64_t* paths, int64_t cap);
 /* dist: int-scaled suffix distance (relax_bu_step, gpucpg.cu:1167-1187),
  * succ: update_successors_bu (gpucpg.cu:2606-2628), level: peel depth
  * (bfs_td_step_privatized, gpucpg.cu:2215-2283).  Runs the SFXT if needed. */
+/* CSR reorder of the resident graph (CsrReorderMethod, gpucpg.cuh:41-72):
+ * 0 NONE, 1 E_ORIENTED -- vertices relabelled in level order (sinks first)
+ * on the device by the edge-parallel scatter of reorder_csr_e_oriented
+ * (gpucpg/gpucpg.cu:2048-2079, run at :17753-17765).  A layout change only:
+ * ties still break on original ids and every id the engine returns is an
+ * original id, so results equal those with 0.  Applied at the next upload
+ * (one SFXT for the levels, then the scatter); cpg_reorder_ms reports its
+ * time.  The BVSS entry points need 0. */
+int cpg_set_csr_reorder(cpg_handle* h, int method);
+double cpg_reorder_ms(cpg_handle* h);
 int cpg_sfxt(cpg_handle* h);
 int cpg_get_sfxt(cpg_handle* h, int* dist, int* succ, int* level);
 
@@ -204,6 +214,24 @@ int cpg_batch_add_edges(cpg_batch* b, int n, int64_t m, const int* eu, const int
                         const float* ew, int64_t k);
 int cpg_batch_num_queries(cpg_batch* b);
 int cpg_batch_run(cpg_batch* b, const cpg_query_opts* o);
+/* Round shapes of a batch, chosen per workload by cpg_batch_autotune (no
+ * counterpart in the reference, which runs one query at a time).  None of
+ * them changes a result. */
+typedef struct cpg_batch_tuning {
+  int group;          /* replayed rounds per host synchronisation (default 4) */
+  int grid_mult;      /* round kernels' grid = CUs x grid_mult blocks (8)     */
+  int target_div[3];  /* per K class (K < 1000, < 10000, >= 10000): a query
+                         expands about K / div parents per round (8, 8, 8)    */
+  int trials;         /* batch runs the last autotune made                    */
+  double start_ms;    /* last autotune: batch ms with the shapes it began from */
+  double best_ms;     /* last autotune: batch ms with the shapes it kept       */
+} cpg_batch_tuning;
+/* Coordinate descent over the shapes: whole batch runs (at most max_trials),
+ * a shape kept when it is >= 2% faster; leaves the chosen shapes set and
+ * the batch's results computed with them.  Returns the trials made. */
+int cpg_batch_autotune(cpg_batch* b, const cpg_query_opts* o, int max_trials);
+int cpg_batch_get_tuning(cpg_batch* b, cpg_batch_tuning* out);
+int cpg_batch_set_tuning(cpg_batch* b, const cpg_batch_tuning* t);
 /* First k costs of query q (ascending, exact), as cpg_get_costs. */
 int64_t cpg_batch_get_costs(cpg_batch* b, int q, int64_t k, float* out);
 /* Whole-batch timings (sfxt/pfxt/select/query ms, rounds, generated). */

@@ -95,12 +95,21 @@ class CpGen {
                     const bool /*enable_runtime_log_file*/ = false,
                     const float init_split_perc = 0.005f, const float /*alpha*/ = 5.0f,
                     const int /*per_thread_work_items*/ = 8, bool /*enable_reindex_cpu*/ = false,
-                    bool /*enable_reindex_gpu*/ = false, bool /*enable_fuse_steps*/ = false,
+                    bool enable_reindex_gpu = false, bool /*enable_fuse_steps*/ = false,
                     bool enable_interm_perf_log = false,
-                    const CsrReorderMethod /*csr_reorder_method*/ = CsrReorderMethod::E_ORIENTED,
+                    const CsrReorderMethod csr_reorder_method = CsrReorderMethod::E_ORIENTED,
                     bool /*enable_tile_spur*/ = false,
                     std::optional<float> /*fixed_split_inc_amount*/ = std::nullopt,
                     std::optional<std::string> /*rabbit_vmap_file*/ = std::nullopt) {
+    // The reference relabels the CSR only with enable_reindex_gpu, and its
+    // V_/E_ORIENTED kernels all build the same level-order CSR
+    // (gpucpg/gpucpg.cu:17690-17781); RABBIT/GORDER/CORDER orders are not
+    // built here and run unrelabelled.  Results are the same either way.
+    const bool level_order = enable_reindex_gpu && (csr_reorder_method == CsrReorderMethod::V_ORIENTED ||
+                                                    csr_reorder_method == CsrReorderMethod::V_ORIENTED_TILE_SCAN ||
+                                                    csr_reorder_method == CsrReorderMethod::E_ORIENTED ||
+                                                    csr_reorder_method == CsrReorderMethod::E_ORIENTED_VEC2);
+    check(cpg_set_csr_reorder(h_, level_order ? 1 : 0));
     cpg_query_opts o;
     cpg_query_opts_default(&o);
     o.init_split_perc = init_split_perc;

@@ -188,6 +188,9 @@ inline T atomicMin(T* p, T v) {
 #define __hip_atomic_fetch_add(p, v, order, scope) __atomic_fetch_add((p), (v), (order))
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 #define __builtin_amdgcn_s_sleep(n) simt::relax()
+// a __device__ variable is host memory here: its address is its own
+#define HIP_SYMBOL(x) (&(x))
+#define __align__(n) __attribute__((aligned(n)))
 // waiting for a thread's outstanding memory operations: on the CPU they are
 // done once issued, so only the compiler's ordering is kept
 #define __builtin_amdgcn_s_waitcnt(n) __atomic_thread_fence(__ATOMIC_SEQ_CST)
@@ -221,6 +224,10 @@ hipError_t hipMalloc(T** p, size_t bytes) {
 }
 hipError_t hipFree(void* p);
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k, hipStream_t s);
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
 hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind k);
 hipError_t hipMemsetAsync(void* p, int v, size_t bytes, hipStream_t s);
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned flags);

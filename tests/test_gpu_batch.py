@@ -97,3 +97,36 @@ def test_full_256_design_batch_equals_oracle():
         assert not bad, bad[:8]
     finally:
         b.close()
+
+
+def test_batch_autotune_keeps_results_exact():
+    """cpg_batch_autotune times whole batch runs over the round shapes and
+    keeps the fastest; the results under the chosen shapes still equal
+    the oracle's, and out-of-range shapes are refused."""
+    designs = [cpgamd.gen_preset(4, i) + (int(cpgamd.preset_k(4, i)),) for i in range(16, 40)]
+    print(f"case batch autotune over {len(designs)} designs", flush=True)
+    b = cpgamd.Batch()
+    try:
+        for n, u, v, w, k in designs:
+            b.add(n, u, v, w, k)
+        t = b.autotune(max_trials=8)
+        print(f"tuning {t}", flush=True)
+        assert 1 <= t["trials"] <= 8 and t["best_ms"] > 0.0 and t["best_ms"] <= t["start_ms"] * 1.0001
+        assert t["group"] in (2, 4, 8, 16) and t["grid_mult"] in (4, 8, 16)
+        for q, (n, u, v, w, k) in enumerate(designs):
+            og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+            dist, succ, _, _ = og.sfxt()
+            want, _ = og.topk(dist, succ, k)
+            got = b.get_slacks(q, k)
+            assert len(got) == len(want) and pyoracle.first_bit_mismatch(want, got) == -1, q
+        b.set_tuning(group=2, grid_mult=16, target_div=(2, 32, 4))
+        b.run()
+        for q, (n, u, v, w, k) in enumerate(designs[:6]):
+            og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+            dist, succ, _, _ = og.sfxt()
+            want, _ = og.topk(dist, succ, k)
+            assert pyoracle.first_bit_mismatch(want, b.get_slacks(q, k)) == -1
+        with pytest.raises(RuntimeError):
+            b.set_tuning(group=0)
+    finally:
+        b.close()
