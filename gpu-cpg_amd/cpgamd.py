@@ -110,6 +110,9 @@ ABI = {
     "cpg_set_csr_reorder": (ctypes.c_int, [_H, ctypes.c_int]),
     "cpg_reorder_ms": (ctypes.c_double, [_H]),
     "cpg_sfxt": (ctypes.c_int, [_H]),
+    "cpg_sfxt_noalloc": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int]
+                         + [ctypes.c_void_p] * 4),
+    "cpg_sfxt_noalloc_max_verts": (ctypes.c_int, []),
     "cpg_get_sfxt": (ctypes.c_int, [_H, _PI, _PI, _PI]),
     "cpg_batch_create": (_H, []),
     "cpg_batch_destroy": (None, [_H]),

@@ -2497,6 +2497,17 @@ int cpg_set_csr_reorder(cpg_handle* h, int method) {
 
 double cpg_reorder_ms(cpg_handle* h) { return h ? h->reorder_ms : -1.0; }
 
+int cpg_sfxt_noalloc(int n, const int* d_oadjp, const int* d_iadjp, const int* d_iedge,
+                     const int* d_sinks, int nsinks, int* d_dist, int* d_succ, int* d_level,
+                     void* stream) {
+  return guarded<int>(-1, [&]() {
+    return sfxt_noalloc(n, d_oadjp, d_iadjp, reinterpret_cast<const int2*>(d_iedge), d_sinks, nsinks,
+                        d_dist, d_succ, d_level, static_cast<hipStream_t>(stream));
+  });
+}
+
+int cpg_sfxt_noalloc_max_verts(void) { return sfxt_noalloc_max_verts(); }
+
 int cpg_sfxt(cpg_handle* h) {
   CPG_CHECK_HANDLE(h, -1)
   return guarded<int>(-1, [&]() {

@@ -338,4 +338,13 @@ void launch_reorder_fanin(const int* iadjp, const int2* iedge, const int* owner,
 // out[i] = in[idx[i]] for bytes (endpoint masks into relabelled order)
 void launch_gather_u8(const uint8_t* in, const int* idx, int n, uint8_t* out, hipStream_t stream);
 
+// ---------------- SFXT without allocation (noalloc.hip) ----------------
+// Levelize + relax + successor of a device-resident graph with every
+// intermediate buffer in a static device array; synchronous on `stream`.
+// Returns the number of levels; throws on a bad argument, a graph larger
+// than sfxt_noalloc_max_verts() or an incomplete loop.
+int sfxt_noalloc_max_verts();
+int sfxt_noalloc(int n, const int* oadjp, const int* iadjp, const int2* iedge, const int* sinks,
+                 int nsinks, int* dist, int* succ, int* level, hipStream_t stream);
+
 }  // namespace cpg

@@ -174,6 +174,22 @@ int64_t cpg_step_times(cpg_handle* h, double* ms, int64_t* paths, int64_t cap);
 int cpg_set_csr_reorder(cpg_handle* h, int method);
 double cpg_reorder_ms(cpg_handle* h);
 int cpg_sfxt(cpg_handle* h);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops or harnesses that forbid hipMalloc): the SFXT level
+ * loop (levelize, int-scaled relax, successor; the same dist/succ/level as
+ * cpg_sfxt + cpg_get_sfxt) of a graph already on the device, with every
+ * intermediate buffer in one static device array (about 24 MiB per device,
+ * held until the process ends), so the call allocates nothing.  Inputs are
+ * device pointers: the fanout offsets (n+1), the fanin offsets (n+1), the
+ * fanin records (2 ints per edge: fanin vertex, weight as float bits) and
+ * the sink list (out-degree-0 vertices, ascending); outputs are n ints each.
+ * At most cpg_sfxt_noalloc_max_verts() vertices (1,048,576).  Synchronous on
+ * `stream` (a hipStream_t, null = the null stream); calls that overlap on one
+ * device race on the shared array.  Returns the number of levels. */
+int cpg_sfxt_noalloc(int n, const int* d_oadjp, const int* d_iadjp, const int* d_iedge,
+                     const int* d_sinks, int nsinks, int* d_dist, int* d_succ, int* d_level,
+                     void* stream);
+int cpg_sfxt_noalloc_max_verts(void);
 int cpg_get_sfxt(cpg_handle* h, int* dist, int* succ, int* level);
 
 /* ---- BVSS deviation discovery, gpucpg/tc_pfxt_bvss.cuh ---- */
