@@ -235,6 +235,7 @@ struct cpg_handle {
 
   cpg_timing timing{};
   bool kernel_timing = false;
+  bool probe_overflowed = false;  // the last bound probe stopped at a full node store
   EventPair ev_sfxt, ev_levels, ev_count, ev_fill;
   // per-round record (step_times option): an event after every round and
   // the stored-path count the round ended with
@@ -586,6 +587,7 @@ struct cpg_handle {
     std::vector<float> rc;
     for (size_t i = 0; i < sd.size(); ++i)
       if (kp[i]) rc.push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+    probe_overflowed = false;
     const uint64_t nb = rc.size();
     launch_iota(batch.p, nb, grid, stream);
     std::nth_element(rc.begin(), rc.begin() + static_cast<std::ptrdiff_t>(rc.size() / 10), rc.end());
@@ -606,7 +608,10 @@ struct cpg_handle {
       d2h(&pc, pctl.p, 1, stream);
       CPG_HIP(hipStreamSynchronize(stream));
       if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
-      if (pc.overflow) break;
+      if (pc.overflow) {  // the trial bound's children do not fit the store
+        probe_overflowed = true;
+        break;
+      }
       uint64_t roots_le = 0;
       for (float c : rc)
         if (c <= u0 && c != 0.0f) ++roots_le;
@@ -741,6 +746,14 @@ struct cpg_handle {
       uint32_t ukey_host = kKeyInf;
       const double tpr0 = now_ms();
       if (probe) ukey_host = probe_bound(wa, k, kp, sd, minc, init.klo, trace);
+      if (probe && probe_overflowed && attempt < 4 && cap < budget_cap && cap < 0x7fffffffull) {
+        // a trial bound's children filled the store: probe again in a larger
+        // store rather than start the rounds unbounded
+        cap = std::min<uint64_t>({cap * 2, 0x7fffffffull, budget_cap});
+        if (o.mem_budget_bytes > 0) release_pfxt_buffers();
+        timing.retries++;
+        continue;
+      }
       if (trace)
         std::fprintf(stderr, "query phases: sfxt %.3f ms, roots %.3f ms, probe %.3f ms\n", timing.sfxt_ms,
                      tpr0 - tp0, now_ms() - tpr0);
@@ -1510,7 +1523,16 @@ struct cpg_server {
   // in-flight estimate within node_budget, a lone query always admitted.
   static constexpr int64_t kNodesPerK = 8;
   int64_t node_budget = int64_t(128) << 20;
+  uint64_t dead_nodes = 0;          // nodes of finished queries still in the store
   int64_t nodes_held = 0;           // node estimates of the admitted queries
+  static bool trace_on() {
+    static const bool on = std::getenv("CPG_TRACE") != nullptr;
+    return on;
+  }
+  static int trace_level() {
+    static const int lv = std::getenv("CPG_TRACE") ? std::atoi(std::getenv("CPG_TRACE")) : 0;
+    return lv;
+  }
   // a query's node estimate, fixed when it is admitted (released with it)
   int64_t node_estimate(const Query& q) const {
     if (q.node_est > 0) return q.node_est;
@@ -1682,35 +1704,42 @@ struct cpg_server {
       std::lock_guard<std::mutex> lk(stats_mu);
       stats.root_bounds++;
     } else if (nkept > 0 && q.k > 0) {
-      const uint64_t cap = std::min<uint64_t>(
+      uint64_t cap = std::min<uint64_t>(
           std::max<uint64_t>(uint64_t(1) << 22, 16ull * static_cast<uint64_t>(q.k) + nkept + (1u << 20)),
           0x7fffffffull);
-      h.ensure_pfxt_buffers(cap);
-      PfxtCtl init{};
-      init.ukey = kKeyInf;
-      init.klo = klo;
-      init.shift = window_shift(klo, kKeyInf);
-      init.node_cap = cap;
-      h2d(h.pctl.p, &init, 1, h.stream);
-      h.keep.reserve(std::max(1, nsrc));
-      h2d(h.keep.p, kp.data(), nsrc, h.stream);
-      launch_roots(h.srcs.p, h.keep.p, nsrc, aq.dist, aq.distf, h.node_store(), h.pctl.p, h.stream);
-      WalkArgs wa{};
-      wa.vrec = aq.vrec; wa.devinfo = aq.devinfo; wa.devs = aq.devs; wa.batch = h.batch.p;
-      wa.nodes = h.node_store(); wa.ckeys = h.ckeys.p; wa.ckeys_cap = cap; wa.ctl = h.pctl.p;
-      wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = h.g.max_abs_wgt;
-      wa.step_limit = h.step_limit();
-      const double tp = now_ms();
-      u0 = h.probe_bound(wa, q.k, kp, sd, minc, klo, false);
-      {
-        std::lock_guard<std::mutex> lk(stats_mu);
-        stats.probe_ms += now_ms() - tp;
+      // a probe whose trial bound's children fill the handle's store runs
+      // again in a larger one: without a bound the first round would expand
+      // every root under a coarse bound and store far more
+      for (int attempt = 0;; ++attempt) {
+        h.ensure_pfxt_buffers(cap);
+        PfxtCtl init{};
+        init.ukey = kKeyInf;
+        init.klo = klo;
+        init.shift = window_shift(klo, kKeyInf);
+        init.node_cap = cap;
+        h2d(h.pctl.p, &init, 1, h.stream);
+        h.keep.reserve(std::max(1, nsrc));
+        h2d(h.keep.p, kp.data(), nsrc, h.stream);
+        launch_roots(h.srcs.p, h.keep.p, nsrc, aq.dist, aq.distf, h.node_store(), h.pctl.p, h.stream);
+        WalkArgs wa{};
+        wa.vrec = aq.vrec; wa.devinfo = aq.devinfo; wa.devs = aq.devs; wa.batch = h.batch.p;
+        wa.nodes = h.node_store(); wa.ckeys = h.ckeys.p; wa.ckeys_cap = cap; wa.ctl = h.pctl.p;
+        wa.max_abs_distf = max_abs_distf; wa.max_abs_wgt = h.g.max_abs_wgt;
+        wa.step_limit = h.step_limit();
+        const double tp = now_ms();
+        u0 = h.probe_bound(wa, q.k, kp, sd, minc, klo, false);
+        {
+          std::lock_guard<std::mutex> lk(stats_mu);
+          stats.probe_ms += now_ms() - tp;
+        }
+        PfxtCtl pc{};
+        d2h(&pc, h.pctl.p, 1, h.stream);
+        CPG_HIP(hipStreamSynchronize(h.stream));
+        if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
+        q.seed_n = std::min<uint64_t>(pc.node_n, cap);
+        if (!(u0 == kKeyInf && h.probe_overflowed) || attempt >= 4 || cap >= 0x7fffffffull) break;
+        cap = std::min<uint64_t>(cap * 2, 0x7fffffffull);
       }
-      PfxtCtl pc{};
-      d2h(&pc, h.pctl.p, 1, h.stream);
-      CPG_HIP(hipStreamSynchronize(h.stream));
-      if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
-      q.seed_n = std::min<uint64_t>(pc.node_n, cap);
     }
     MqQuery m{};
     m.ukey = u0;
@@ -2042,6 +2071,7 @@ struct cpg_server {
     CPG_HIP(hipStreamSynchronize(stream));
     for (int sl : dead_slots) free_slots.push_back(sl);
     dead_slots.clear();
+    dead_nodes = 0;
     stats.compactions++;
     stats.compact_ms += now_ms() - t0;
   }
@@ -2182,6 +2212,12 @@ struct cpg_server {
         const double used = static_cast<double>(hq[sl].stored + q.seed_n + q.roots.size());
         const double r = used / static_cast<double>(std::max<int64_t>(q.k, 1));
         Gq.nodes_per_k = 0.5 * Gq.nodes_per_k + 0.5 * std::max(1.0, r);
+        dead_nodes += static_cast<uint64_t>(used);
+        if (trace_on())
+          std::fprintf(stderr, "served query %lld graph %d k %lld stored %llu seed %llu roots %zu ratio %.1f\n",
+                       static_cast<long long>(q.id), q.graph, static_cast<long long>(q.k),
+                       static_cast<unsigned long long>(hq[sl].stored),
+                       static_cast<unsigned long long>(q.seed_n), q.roots.size(), r);
       }
       q.result.assign(all.begin() + off, all.begin() + off + take);
       off += pq[sl];
@@ -2223,6 +2259,7 @@ struct cpg_server {
     h2d(qs.p, hq.data(), nslots, stream);
     free_slots.clear();
     dead_slots.clear();
+    dead_nodes = 0;
     for (int i = nslots - 1; i >= 0; --i)
       if (!slot_reserved[i]) free_slots.push_back(i);
     const uint64_t c = std::min<uint64_t>(2 * st[cur].cap, 0x7fffffffull);
@@ -2255,6 +2292,19 @@ struct cpg_server {
       stats.nonmonotone = nonmono_base + static_cast<int64_t>(mc.nonmono);
       node_n = mc.node_n;
       stats.peak_nodes = std::max<int64_t>(stats.peak_nodes, static_cast<int64_t>(std::min(node_n, st[cur].cap)));
+      if (trace_level() >= 2)
+        for (int sl = 0; sl < nslots; ++sl)
+          if (slot_query[sl] >= 0)
+            std::fprintf(stderr, "  slot %d query %lld U %.9g th %.9g stored %llu done %d rounds %d batch %llu\n", sl,
+                         static_cast<long long>(slot_query[sl]), static_cast<double>(key_float(hq[sl].ukey)),
+                         static_cast<double>(key_float(hq[sl].thkey)),
+                         static_cast<unsigned long long>(hq[sl].stored), hq[sl].done, hq[sl].rounds,
+                         static_cast<unsigned long long>(mc.batch_n));
+      if (trace_on() && (stats.rounds % 16 == 0 || mc.overflow || trace_level() >= 2))
+        std::fprintf(stderr, "served round %lld node_n %llu cap %llu in_flight %d queued %zu dead %zu held %lld ms %.2f%s\n",
+                     static_cast<long long>(stats.rounds), static_cast<unsigned long long>(node_n),
+                     static_cast<unsigned long long>(st[cur].cap), in_flight, queue.size(), dead_slots.size(),
+                     static_cast<long long>(nodes_held), now_ms() - t0, mc.overflow ? " OVERFLOW" : "");
       if (mc.overflow) {
         restart_all();
       } else {
@@ -2263,9 +2313,11 @@ struct cpg_server {
           if (slot_query[sl] >= 0 && hq[sl].done) fin.push_back(sl);
         if (!fin.empty()) extract(fin);
         // every round scans the whole store: drop finished queries' nodes
-        // once they may outweigh the live ones (or the store is half full)
+        // once they are a quarter of it (or the store is half full, or as
+        // many queries finished as are in flight)
         if (!dead_slots.empty() && node_n > (uint64_t(1) << 22) &&
-            (node_n > st[cur].cap / 2 || static_cast<int>(dead_slots.size()) >= std::max(1, in_flight)))
+            (node_n > st[cur].cap / 2 || dead_nodes * 4 > node_n ||
+             static_cast<int>(dead_slots.size()) >= std::max(1, in_flight)))
           compact(st[cur].cap);
       }
     }
