@@ -6,8 +6,10 @@ and 300 levels (out-degree geometric, mean 2.7; lognormal(-2, 0.7) delays;
 A step is one complete query with the graph already resident in HBM:
 SFXT (levelize, relax, successor, deviation index) + PFXT expansion + final
 top-K selection, and the K costs copied to host memory (get_slacks).  The
-static cache is off, so every
-step recomputes the SFXT.
+static cache is off, so every step recomputes the SFXT.  The resident graph
+is relabelled in level order (the E_ORIENTED CSR reorder) once, before the
+timed steps, and reported as reorder_ms; --no-reorder keeps the input order.
+Results are identical either way.
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline        the dominant kernel's achieved algorithmic HBM bandwidth,
@@ -79,7 +81,7 @@ def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by a copy-kernel calibration),
     or None when no summary covers it."""
-    path = os.path.join(ROOT, "profiles", "traffic_sfxt_levels_walk_c2_k1m.md")
+    path = os.path.join(ROOT, "profiles", "traffic_c2_k1m.md")
     try:
         with open(path) as f:
             for line in f:
@@ -413,8 +415,11 @@ def main():
     #  child 25 B (node slot)
     walk_bytes = 16.0 * st["hops"] + 16.0 * st["dev_visits"] + 12.0 * st["expanded"] \
         + 25.0 * (st["generated"] - st["roots"])
+    # the level loop's variant with 4 records per lane step runs where the
+    # graph has fewer than 4 edges per vertex, the 8-record one otherwise
+    lvl_kernel = "sfxt_levels_nf4_kernel" if m < 4 * n else "sfxt_levels_nf_kernel"
     kernels = {
-        "sfxt_levels_nf_kernel": (levels_ms, levels_bytes, 1),
+        lvl_kernel: (levels_ms, levels_bytes, 1),
         "walk_kernel": (walk_ms, walk_bytes, max(walk_launches, 1)),
     }
     dom = max(kernels, key=lambda name: kernels[name][0])

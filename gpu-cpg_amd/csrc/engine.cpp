@@ -168,7 +168,7 @@ struct cpg_handle {
   DevBuf<uint8_t> emask;
   int diameter = 0;
   int sfxt_blocks = 0;       // level-loop grid requested (0 = automatic)
-  int sfxt_method = 0;       // 0 automatic, 1 level loop, 2 dataflow loop, 3 local-queue level loop
+  int sfxt_method = 0;       // 0 automatic, 1 level loop, 2 dataflow, 3 local queues, 4 fence-free, 5 edge-parallel
   int last_diameter = 0;     // levels of the last SFXT on this graph (automatic grid)
   float max_abs_distf = 0.0f;
   bool cache_enabled = false;
@@ -432,9 +432,10 @@ struct cpg_handle {
     if (cache_enabled && count_cache) ++cache_misses;
     const double t0 = now_ms();
     const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
-    deps.reserve(n); queue.reserve(n); lvlp.reserve(n + 2); level.reserve(n);
+    // best[] and deps[] carry the level loops' pad slots for no-op atomics
+    deps.reserve(n + kSfxtPad); queue.reserve(n); lvlp.reserve(n + 2); level.reserve(n);
     dist.reserve(n); succ.reserve(n); distf.reserve(n); vrec.reserve(n);
-    devinfo.reserve(n); devs.reserve(m); big.reserve(n); best.reserve(n); sctl.reserve(1);
+    devinfo.reserve(n); devs.reserve(m); big.reserve(n); best.reserve(n + kSfxtPad); sctl.reserve(1);
     if (g.max_out_degree > kSfxtLdsSort) devs_tmp.reserve(m);
     CPG_HIP(hipMemsetAsync(sctl.p, 0, sizeof(SfxtCtl), stream));
     SfxtCtl c{};
@@ -462,15 +463,14 @@ struct cpg_handle {
         a.sink_mask = emask.p;
       }
       a.orig = relabelled ? orig_d.p : nullptr;
-      // 0 (automatic) is the fence-free level loop with its working blocks
-      // on one XCD (round-robin dispatch puts every 8th block on the same
-      // XCD): 3.67 ms on the 1.5M-vertex DAG against 4.2 ms for the fenced
-      // loop over all XCDs
-      const int method = sfxt_method == 0 ? 4 : sfxt_method;
+      // 0 (automatic) is the edge-parallel fence-free level loop with its
+      // working blocks on one XCD (round-robin dispatch puts every 8th block
+      // on the same XCD)
+      const int method = sfxt_method == 0 ? 5 : sfxt_method;
       const bool dataflow = method == 2;
       if (dataflow && relabelled)
         throw std::runtime_error("the dataflow SFXT loop does not run on a relabelled graph");
-      a.place_stride = sfxt_method == 0 ? kXcds : 1;
+      a.place_stride = method == 5 ? kXcds : 1;
       int df_grid = 0;
       if (dataflow) {
         df_grid = sfxt_dataflow_grid(sfxt_blocks);
@@ -483,13 +483,14 @@ struct cpg_handle {
         ev_levels.create();
         CPG_HIP(hipEventRecord(ev_sfxt.a, stream));
       }
-      if (dataflow || method == 4) launch_sfxt_dataflow_init(a, stream);
+      if (dataflow || method >= 4) launch_sfxt_dataflow_init(a, stream);
       else if (method == 3) launch_sfxt_local_init(a, stream);
       else launch_sfxt_init(a, stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.a, stream));
       timing.sfxt_blocks = dataflow      ? launch_sfxt_dataflow(a, df_grid, stream)
                            : method == 3 ? launch_sfxt_local(a, level_loop_blocks(), stream)
                            : method == 4 ? launch_sfxt_levels_nf(a, level_loop_blocks(), stream)
+                           : method == 5 ? launch_sfxt_levels_edges(a, level_loop_blocks(), stream)
                                          : launch_sfxt_levels(a, level_loop_blocks(), stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.b, stream));
       if (relabelled) launch_sfxt_succ_relabel(succ.p, perm_d.p, g.n, stream);
@@ -588,10 +589,12 @@ struct cpg_handle {
     for (size_t i = 0; i < sd.size(); ++i)
       if (kp[i]) rc.push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
     probe_overflowed = false;
+    const double tb0 = now_ms();
     const uint64_t nb = rc.size();
     launch_iota(batch.p, nb, grid, stream);
     std::nth_element(rc.begin(), rc.begin() + static_cast<std::ptrdiff_t>(rc.size() / 10), rc.end());
     const float p10 = rc[rc.size() / 10];
+    if (trace) std::fprintf(stderr, "probe: host preparation %.3f ms\n", now_ms() - tb0);
     double gap = static_cast<double>(p10) - static_cast<double>(cmin);
     gap = std::max({gap, std::fabs(static_cast<double>(cmin)) * 0x1p-8,
                     static_cast<double>(max_abs_distf) * 0x1p-8, 1e-6});
@@ -669,7 +672,7 @@ struct cpg_handle {
     step_ms.clear();
     step_paths.clear();
     if (o.sfxt_blocks < 0) throw std::runtime_error("sfxt_blocks must be >= 0");
-    if (o.sfxt_method < 0 || o.sfxt_method > 4) throw std::runtime_error("sfxt_method must be 0-4");
+    if (o.sfxt_method < 0 || o.sfxt_method > 5) throw std::runtime_error("sfxt_method must be 0-5");
     sfxt_blocks = o.sfxt_blocks;
     sfxt_method = o.sfxt_method;
     const double tq0 = now_ms();
@@ -680,6 +683,7 @@ struct cpg_handle {
       return;
     }
     const double tp0 = now_ms();
+    const bool trace = std::getenv("CPG_TRACE") != nullptr;
     // roots: sources with finite dist (gpucpg/gpucpg.cu:18502-18513)
     const int nsrc = static_cast<int>(g.srcs.size());
     std::vector<int> sd(nsrc);
@@ -687,10 +691,14 @@ struct cpg_handle {
     launch_gather_int(dist.p, srcs.p, nsrc, src_dist.p, stream);
     d2h(sd.data(), src_dist.p, nsrc, stream);
     CPG_HIP(hipStreamSynchronize(stream));
+    const double tr_copy = now_ms();
     std::vector<uint8_t> kp;
     int64_t nroots = 0, dropped = 0;
     float minc = 0.0f;
     pick_roots(sd, o.root_tie_quirk != 0, o.init_split_perc, kp, nroots, dropped, minc);
+    if (trace)
+      std::fprintf(stderr, "query roots: source distances to host %.3f ms, pick %.3f ms (%d sources)\n",
+                   tr_copy - tp0, now_ms() - tr_copy, nsrc);
     timing.roots = nroots - dropped;
     timing.roots_dropped = dropped;
     keep.reserve(std::max(1, nsrc));
@@ -717,7 +725,6 @@ struct cpg_handle {
       if (node_cap > cap) release_pfxt_buffers();
     }
     PfxtCtl c{};
-    const bool trace = std::getenv("CPG_TRACE") != nullptr;
     // Rounds with a finite bound walk once: the fill pass stores children at
     // the bound, their histogram joins the stored paths', and the bound is
     // tightened.  A node-store overflow restarts the query with a larger
@@ -745,6 +752,7 @@ struct cpg_handle {
                                        (o.bound_probe == 0 && nkept * (diameter + 1) > (1 << 20)));
       uint32_t ukey_host = kKeyInf;
       const double tpr0 = now_ms();
+      if (trace) std::fprintf(stderr, "query roots: buffers, flags and root launch %.3f ms\n", tpr0 - tp0);
       if (probe) ukey_host = probe_bound(wa, k, kp, sd, minc, init.klo, trace);
       if (probe && probe_overflowed && attempt < 4 && cap < budget_cap && cap < 0x7fffffffull) {
         // a trial bound's children filled the store: probe again in a larger
@@ -1246,7 +1254,7 @@ struct cpg_batch {
     const double t0 = now_ms();
     uh.kernel_timing = false;
     if (o.sfxt_blocks < 0) throw std::runtime_error("sfxt_blocks must be >= 0");
-    if (o.sfxt_method < 0 || o.sfxt_method > 4) throw std::runtime_error("sfxt_method must be 0-4");
+    if (o.sfxt_method < 0 || o.sfxt_method > 5) throw std::runtime_error("sfxt_method must be 0-5");
     uh.sfxt_blocks = o.sfxt_blocks;
     uh.sfxt_method = o.sfxt_method;
     uh.reset();
@@ -1815,7 +1823,7 @@ struct cpg_server {
         q.node_est = node_estimate(q);
         nodes_held += q.node_est;
       }
-      h.deps.reserve(n); h.queue.reserve(n); h.lvlp.reserve(n + 2); h.level.reserve(n);
+      h.deps.reserve(n + kSfxtPad); h.queue.reserve(n); h.lvlp.reserve(n + 2); h.level.reserve(n);
       h.big.reserve(n); h.sctl.reserve(1);
       if (h.g.max_out_degree > kSfxtLdsSort) h.devs_tmp.reserve(m);
       Graph::Multi& M = G.mm;

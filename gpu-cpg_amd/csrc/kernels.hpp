@@ -21,6 +21,9 @@ struct SfxtCtl {
   unsigned long long nbig_l;    // deviation lists one block sorts (back of big[])
   unsigned qhead;               // dataflow loop: queue slots claimed
   int pad;
+  // edge-parallel level loop: one counter per level parity, arrivals << 40 |
+  // vertices pushed (both running totals, never reset)
+  unsigned long long lvl_bar[2];
 };
 
 struct SfxtArgs {
@@ -105,6 +108,9 @@ int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream);
 // the queue must hold the sinks and -1 beyond: launch_sfxt_dataflow_init
 // with args.dfw_n = 0 prepares it.
 int launch_sfxt_levels_nf(const SfxtArgs& args, int blocks, hipStream_t stream);
+// The fence-free loop with the fanin release spread over a wave's records
+// (same preparation as launch_sfxt_levels_nf).
+int launch_sfxt_levels_edges(const SfxtArgs& args, int blocks, hipStream_t stream);
 // The barrier-free dataflow loop (same results as the level loop); args.dfw
 // must hold at least sfxt_dataflow_grid(blocks) words.  Returns the grid used.
 int sfxt_dataflow_grid(int blocks);
@@ -121,6 +127,10 @@ void launch_sfxt_succ_relabel(int* succ, const int* perm, int n, hipStream_t str
 // m: the graph's edge count (chooses thread- or wave-per-vertex records).
 void launch_sfxt_index(const SfxtArgs& args, int lift_levels, int64_t m, hipStream_t stream);
 constexpr int kSfxtLdsSort = 2048;  // longest deviation list sorted in LDS
+// best[] and deps[] hold kSfxtPad slots after their n vertices: the level
+// loops aim their no-op atomics there (a slot past a vertex's last fanin
+// record), so those stay in a small range that the L2 keeps
+constexpr int kSfxtPad = 1 << 16;
 
 // ---------------- PFXT ----------------
 constexpr int kHistParts = 256;  // per-block partial histograms per pass

@@ -24,14 +24,15 @@ namespace {
 constexpr int kNoallocMaxVerts = 1 << 20;  // 1,048,576 vertices
 constexpr size_t kAlign = 256;
 constexpr size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
-// layout: control block, best (8 B), deps, queue, distf (4 B each), lvlp (n + 2 ints)
+// layout: control block, best (8 B), deps, queue, distf (4 B each), lvlp
+// (n + 2 ints); best and deps carry the level loop's kSfxtPad no-op slots
 constexpr size_t kOffCtl = 0;
 constexpr size_t kOffBest = align_up(sizeof(SfxtCtl));
-constexpr size_t kOffDeps = kOffBest + align_up(8ull * kNoallocMaxVerts);
-constexpr size_t kOffQueue = kOffDeps + align_up(4ull * kNoallocMaxVerts);
+constexpr size_t kOffDeps = kOffBest + align_up(8ull * (kNoallocMaxVerts + kSfxtPad));
+constexpr size_t kOffQueue = kOffDeps + align_up(4ull * (kNoallocMaxVerts + kSfxtPad));
 constexpr size_t kOffDistf = kOffQueue + align_up(4ull * kNoallocMaxVerts);
 constexpr size_t kOffLvlp = kOffDistf + align_up(4ull * kNoallocMaxVerts);
-constexpr size_t kScratchBytes = kOffLvlp + align_up(4ull * (kNoallocMaxVerts + 2));  // ~24 MiB
+constexpr size_t kScratchBytes = kOffLvlp + align_up(4ull * (kNoallocMaxVerts + 2));  // ~25 MiB
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -75,7 +76,7 @@ int sfxt_noalloc(int n, const int* oadjp, const int* iadjp, const int2* iedge, c
   unsigned char* base = scratch_ptr();
   SfxtArgs a{};
   a.n = n;
-  a.m = 0;  // records per step: 4 (the sparse case); the result does not depend on it
+  a.m = 0;  // list positions per lane step: 3 (the sparse case); the result does not depend on it
   a.nsinks = nsinks;
   a.oadjp = oadjp;
   a.iadjp = iadjp;
@@ -94,7 +95,7 @@ int sfxt_noalloc(int n, const int* oadjp, const int* iadjp, const int2* iedge, c
   hipLaunchKernelGGL(noalloc_clear_ctl_kernel, dim3(1), dim3(64), 0, stream, a.ctl);
   CPG_HIP(hipGetLastError());
   launch_sfxt_dataflow_init(a, stream);  // counters, candidates, queue = sinks then -1
-  launch_sfxt_levels_nf(a, kNoallocBlocks, stream);
+  launch_sfxt_levels_edges(a, kNoallocBlocks, stream);
   SfxtCtl c{};
   CPG_HIP(hipMemcpyAsync(&c, a.ctl, sizeof(SfxtCtl), hipMemcpyDeviceToHost, stream));
   CPG_HIP(hipStreamSynchronize(stream));

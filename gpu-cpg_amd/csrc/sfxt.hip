@@ -297,17 +297,17 @@ __device__ void levels_loop(const SfxtArgs& a) {
         // Branch-free atomics, all in flight together: with a branch per
         // record the compiler waits for each returning atomic before the
         // next.  A slot past the end issues no-op atomics (min with the
-        // largest key, subtract 0), which change nothing wherever they
-        // land; they land on a vertex spread by thread and slot, since
-        // atomics on one address (the copied record) serialise.
+        // largest key, subtract 0) on a pad slot after the n vertices,
+        // spread by thread and slot (atomics on one address serialise) in
+        // a range small enough to stay in the L2.
         int old[kU];
 #pragma unroll
         for (int q = 0; q < kU; ++q) {
           const bool in = j + q < e;
           const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
           const int x = in ? r[q].x
-                           : static_cast<int>((static_cast<unsigned>(gtid) * kU + static_cast<unsigned>(q)) %
-                                              static_cast<unsigned>(a.n));
+                           : a.n + static_cast<int>((static_cast<unsigned>(gtid) * kU + static_cast<unsigned>(q)) &
+                                                    static_cast<unsigned>(kSfxtPad - 1));
           atomicMin(&a.best[x], in && reach ? pack_cand(du + wi, tie) : ~0ull);
           old[q] = atomicSub(&a.deps[x], in ? 1 : 0);
         }
@@ -340,6 +340,227 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) { l
 // DAG: 11.5 ms against 10.4 ms).
 __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf_kernel(SfxtArgs a) { levels_loop<false, 8>(a); }
 __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf4_kernel(SfxtArgs a) { levels_loop<false, 4>(a); }
+
+// Edge-parallel fence-free level loop: the same peel, relax and successor
+// as levels_loop<false>, with the fanin release spread over the records of
+// a wave's vertices instead of over one vertex per lane.
+//
+// A wave takes 64 queue entries at a time (one per lane), settles them as
+// above, and prefix-sums their in-degrees over the wave; the wave's records
+// then form one flat list, handed out kS per lane step (lane l takes list
+// positions r0 + s*64 + l), and a lane finds the vertex owning its position
+// by a binary search over the exclusive prefix sums held in the lanes
+// (ds_bpermute).  Every device-scope atomic costs a memory-side write
+// whether it changes anything or not, so the per-lane loop's no-op slots (a
+// vertex's record count rounded up to its step) were 40% of that loop's
+// atomics on the 1.5M-vertex DAG; here only the positions past the wave's
+// last record in its final step issue no-ops (on the pad slots).  Records of
+// one vertex sit in consecutive lanes, so their loads coalesce.
+//
+// Level barrier: two 64-bit counters, ctl->lvl_bar[d & 1], each holding the
+// arrivals at its levels (top 24 bits) and the vertices pushed at them (low
+// 40 bits), both running totals that are never reset.  A block arrives with
+// one returning atomic that adds its LDS queue count; the value it returns
+// is also where the block's entries go: the level's entries before it are
+// the counter's pushes since this parity's previous level (last_low), so the
+// block writes its entries at le + (old - last_low) after arriving, while
+// the others are still on their way (the next level's readers wait on a
+// slot that is still -1).  A vertex pushed past the LDS queue reserves its
+// slot the same way, with a pushes-only add.  The waiters poll the counter
+// itself until every working block has arrived; the last to arrive sees that
+// in its own atomic's result.  So a level ends after one returning atomic
+// and, for the others, the polls that see it, where the generation barrier
+// also needed a queue-tail atomic, a store-completion wait before arriving
+// and the last block's release store.
+//
+// Two counters make the running totals safe to read: a block that has seen
+// level d complete cannot be overtaken at level d + 2 (the same counter)
+// before it has itself arrived at level d + 1, so the value it reads is level
+// d's final one.  Arrival counts are compared modulo 2^24.
+template <int kS>
+__device__ void levels_edges_loop(const SfxtArgs& a) {
+  __shared__ int s_q[kLdsQueue];
+  __shared__ int s_qn;
+  __shared__ unsigned s_base;
+  __shared__ unsigned long long s_low;
+  __shared__ int s_ok;
+  constexpr unsigned long long kLow = (1ull << 40) - 1;
+  SfxtCtl* ctl = a.ctl;
+  const unsigned stride = a.place_stride > 1 ? static_cast<unsigned>(a.place_stride) : 1u;
+  if (blockIdx.x % stride != 0) return;  // block-uniform: not a working block
+  const unsigned nblocks = gridDim.x / stride;
+  const int gtid = static_cast<int>(blockIdx.x / stride * blockDim.x + threadIdx.x);
+  const int lane = __lane_id();
+  const unsigned wave = static_cast<unsigned>(gtid) >> 6;
+  const unsigned nwaves = nblocks * blockDim.x / 64u;
+  if (threadIdx.x == 0) s_qn = 0;
+  __syncthreads();
+
+  unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
+  int d = 0;
+  // pushes counted on each parity's counter up to its previous level (two
+  // scalars, not an indexed array, so that they stay in registers)
+  unsigned long long last_low0 = 0ull, last_low1 = 0ull;
+  auto push = [&](int x) {
+    const int p = atomicAdd(&s_qn, 1);
+    if (p < kLdsQueue) {
+      s_q[p] = x;
+    } else {  // past the LDS queue: reserve a slot of this level directly
+      const unsigned long long old =
+          __hip_atomic_fetch_add(&ctl->lvl_bar[d & 1], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long last = (d & 1) ? last_low1 : last_low0;
+      st_dev32(a.queue + le + static_cast<unsigned>((old & kLow) - last), x);
+    }
+  };
+  // Arrives at level d's barrier, writes the block's entries and waits for
+  // the other blocks; returns false on error, else the next level's end.
+  auto level_barrier = [&](unsigned* next_end) -> bool {
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's atomics and stores are done
+    __syncthreads();
+    const int cnt = min(s_qn, kLdsQueue);
+    const int par = d & 1;
+    const unsigned long long last = par ? last_low1 : last_low0;
+    unsigned long long mine = 0;
+    if (threadIdx.x == 0) {
+      const unsigned long long add = (1ull << 40) | static_cast<unsigned long long>(cnt);
+      const unsigned long long old =
+          __hip_atomic_fetch_add(&ctl->lvl_bar[par], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_base = le + static_cast<unsigned>((old & kLow) - last);
+      mine = old + add;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) st_dev32(a.queue + s_base + i, s_q[i]);
+    if (threadIdx.x == 0) {
+      s_qn = 0;  // every thread read the count before the barrier above
+      const unsigned target = ((static_cast<unsigned>(d) >> 1) + 1u) * nblocks;
+      unsigned long long w = mine;
+      int ok = 1;
+      unsigned spins = 0;
+      // arrivals so far minus the target, as a signed 24-bit difference
+      auto short_of = [&](unsigned long long v) {
+        const unsigned diff = (static_cast<unsigned>(v >> 40) - target) & 0xffffffu;
+        return (diff & 0x800000u) != 0;
+      };
+      while (short_of(w)) {
+        if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
+        if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); ok = 0; break; }
+        __builtin_amdgcn_s_sleep(1);
+        w = ld_dev64(&ctl->lvl_bar[par]);
+      }
+      s_low = w & kLow;
+      s_ok = ok;
+    }
+    __syncthreads();
+    const unsigned long long low = s_low;
+    *next_end = le + static_cast<unsigned>(low - last);
+    if (par) last_low1 = low;
+    else last_low0 = low;
+    return s_ok != 0;
+  };
+
+  float local_max = 0.0f;
+  while (true) {
+    if (gtid == 0) a.lvlp[d] = static_cast<int>(lb);
+    // wave-uniform loop over 64-entry chunks of the level
+    for (unsigned i0 = lb + wave * 64u; i0 < le; i0 += nwaves * 64u) {
+      const unsigned i = i0 + static_cast<unsigned>(lane);
+      int b = 0, deg = 0, du = kIntMax, tie = 0;
+      if (i < le) {
+        int u = ld_dev32(a.queue + i);
+        unsigned spins = 0;
+        while (u < 0) {  // written before the barrier; a store still landing
+          if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); break; }
+          __builtin_amdgcn_s_sleep(1);
+          u = ld_dev32(a.queue + i);
+        }
+        if (u >= 0) {
+          const unsigned long long key = ld_dev64(a.best + u);
+          b = a.iadjp[u];
+          deg = a.iadjp[u + 1] - b;
+          int s = -1;
+          du = 0;
+          if (d == 0) {
+            if (a.sink_mask && !a.sink_mask[u]) du = kIntMax;
+          } else if (key == ~0ull) {
+            du = kIntMax;
+          } else {
+            du = unpack_cand(key);
+            s = unpack_succ(key);
+          }
+          const float df = static_cast<float>(du) / static_cast<float>(kScaleUp);
+          tie = a.orig ? a.orig[u] : u;  // successor tie-break: original id
+          a.dist[u] = du;
+          a.succ[u] = s;
+          a.level[u] = d;
+          a.distf[u] = df;
+          if (du != kIntMax) local_max = fmaxf(local_max, fabsf(df));
+        }
+      }
+      // inclusive prefix sum of the in-degrees over the wave
+      int incl = deg;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+      }
+      const int excl = incl - deg;
+      const int total = __shfl(incl, 63);
+      for (int r0 = 0; r0 < total; r0 += 64 * kS) {
+        int2 rec[kS];
+        int cand_du[kS], cand_tie[kS];
+#pragma unroll
+        for (int s = 0; s < kS; ++s) {
+          // list position, clamped to the last record so that every load
+          // is in range; positions past the end issue no-op atomics below
+          const int r = min(r0 + s * 64 + lane, total - 1);
+          int o = 0;  // owner: the last lane whose exclusive sum is <= r
+#pragma unroll
+          for (int step = 32; step > 0; step >>= 1)
+            if (__shfl(excl, o + step) <= r) o += step;
+          const int j = __shfl(b, o) + (r - __shfl(excl, o));
+          rec[s] = a.iedge[j];
+          cand_du[s] = __shfl(du, o);
+          cand_tie[s] = __shfl(tie, o);
+        }
+        int old[kS];
+#pragma unroll
+        for (int s = 0; s < kS; ++s) {
+          // branch-free: every slot issues both atomics, so they are all
+          // in flight together (see levels_loop)
+          const bool in = r0 + s * 64 + lane < total;
+          const bool reach = cand_du[s] != kIntMax;
+          const int wi = static_cast<int>(__int_as_float(rec[s].y) * static_cast<float>(kScaleUp));
+          const int x = in ? rec[s].x
+                           : a.n + static_cast<int>((static_cast<unsigned>(gtid) * kS + static_cast<unsigned>(s)) &
+                                                    static_cast<unsigned>(kSfxtPad - 1));
+          atomicMin(&a.best[x], in && reach ? pack_cand(cand_du[s] + wi, cand_tie[s]) : ~0ull);
+          old[s] = atomicSub(&a.deps[x], in ? 1 : 0);
+        }
+#pragma unroll
+        for (int s = 0; s < kS; ++s)
+          if (r0 + s * 64 + lane < total && old[s] == 1) push(rec[s].x);
+      }
+    }
+    unsigned ne = 0;
+    if (!level_barrier(&ne)) return;
+    if (ne == le) break;
+    lb = le;
+    le = ne;
+    ++d;
+  }
+  const float m = wave_max(local_max);
+  if (__lane_id() == 0) atomicMax(&ctl->max_abs_distf_bits, __float_as_uint(m));
+  if (gtid == 0) {
+    a.lvlp[d + 1] = static_cast<int>(le);
+    ctl->num_levels = le > 0 ? d + 1 : 0;
+    ctl->levelized = le;
+  }
+}
+
+// 3 list positions per lane step where the mean in-degree is under 4 (a
+// wave's 64 vertices then hold about 64 x 2.7 records: one step), 8 above.
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_e3_kernel(SfxtArgs a) { levels_edges_loop<3>(a); }
+__global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_e8_kernel(SfxtArgs a) { levels_edges_loop<8>(a); }
 
 // ---------------------------------------------------------------------------
 // Dataflow loop: the same levelize / relax / successor results as the level
@@ -846,13 +1067,14 @@ __global__ void __launch_bounds__(kLocBlock) sfxt_local_kernel(SfxtArgs a) {
         for (int q = 0; q < kLocUnroll; ++q) {
           xb[q] = a.iadjp[r[q].x];
           xe[q] = a.iadjp[r[q].x + 1];
-          // branch-free: a slot past the end issues no-op atomics on a
-          // vertex spread by thread and slot
+          // branch-free: a slot past the end issues no-op atomics on a pad
+          // slot after the n vertices, spread by thread and slot
           const bool in = j + q < e;
           const int wi = static_cast<int>(__int_as_float(r[q].y) * static_cast<float>(kScaleUp));
           const int x = in ? r[q].x
-                           : static_cast<int>(((bid * kLocBlock + static_cast<unsigned>(t)) * kLocUnroll +
-                                               static_cast<unsigned>(q)) % static_cast<unsigned>(a.n));
+                           : a.n + static_cast<int>(((bid * kLocBlock + static_cast<unsigned>(t)) * kLocUnroll +
+                                                     static_cast<unsigned>(q)) &
+                                                    static_cast<unsigned>(kSfxtPad - 1));
           atomicMin(&a.best[x], in && reach ? pack_cand(du + wi, tie) : ~0ull);
           old[q] = atomicSub(&a.deps[x], in ? 1 : 0);
         }
@@ -1024,17 +1246,28 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
       // release: every lane takes fanin records (the subsets share the
       // levels, so each record is released once)
       for (int j0 = ib; j0 < ie; j0 += 64 * kMultiSteps) {
+        if (ie - j0 <= 64) {
+          // at most one record per lane (u, and so the branch, is uniform
+          // over the wave): one atomic per lane that has a record, no no-ops
+          const int j = j0 + lane;
+          if (j < ie) {
+            const int x = a.iedge[j].x;
+            if (atomicSub(&a.deps[x], 1) == 1) push(x);
+          }
+          continue;
+        }
         int x[kMultiSteps], old[kMultiSteps];
         // branch-free, so the atomics are in flight together: a slot past
-        // the end subtracts 0 (a no-op) from a vertex spread by lane and
-        // step (atomics on one address serialise)
+        // the end subtracts 0 (a no-op) from a pad slot after the n
+        // vertices, spread by lane and step (atomics on one address
+        // serialise)
 #pragma unroll
         for (int t = 0; t < kMultiSteps; ++t) {
           const int j = j0 + t * 64 + lane;
           x[t] = j < ie ? a.iedge[j].x
-                        : static_cast<int>((static_cast<unsigned>(wave) * 64u * kMultiSteps +
-                                            static_cast<unsigned>(t * 64 + lane)) %
-                                           static_cast<unsigned>(a.n));
+                        : a.n + static_cast<int>((static_cast<unsigned>(wave) * 64u * kMultiSteps +
+                                                  static_cast<unsigned>(t * 64 + lane)) &
+                                                 static_cast<unsigned>(kSfxtPad - 1));
         }
 #pragma unroll
         for (int t = 0; t < kMultiSteps; ++t) old[t] = atomicSub(&a.deps[x[t]], j0 + t * 64 + lane < ie ? 1 : 0);
@@ -1409,6 +1642,16 @@ int launch_sfxt_levels_nf(const SfxtArgs& args, int blocks, hipStream_t stream) 
   // mean in-degree m / n: under 4 -> 4 records per step, else 8
   const int64_t m = args.m, n = std::max(1, args.n);
   auto k = m < 4 * n ? sfxt_levels_nf4_kernel : sfxt_levels_nf_kernel;
+  const int grid = placed_grid(blocks, resident_blocks(k, kSfxtBlock), stride);
+  launch_resident(k, grid, kSfxtBlock, args, stream);
+  return grid / stride;
+}
+
+int launch_sfxt_levels_edges(const SfxtArgs& args, int blocks, hipStream_t stream) {
+  const int stride = std::max(1, args.place_stride);
+  // mean in-degree m / n: under 4 -> 3 list positions per lane step, else 8
+  const int64_t m = args.m, n = std::max(1, args.n);
+  auto k = m < 4 * n ? sfxt_levels_e3_kernel : sfxt_levels_e8_kernel;
   const int grid = placed_grid(blocks, resident_blocks(k, kSfxtBlock), stride);
   launch_resident(k, grid, kSfxtBlock, args, stream);
   return grid / stride;

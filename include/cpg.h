@@ -59,13 +59,15 @@ typedef struct {
   int bound_probe;          /* first bound from pruned count passes: 0 = when the
                                roots' chains are long (automatic), 1 = always,
                                -1 = never (the first round walks unbounded)   */
-  int sfxt_method;          /* SFXT loop: 0 automatic (4 with its blocks on one
-                               XCD), 1 level-synchronous loop with a global
+  int sfxt_method;          /* SFXT loop: 0 automatic (5), 1 level-synchronous
+                               loop with a global
                                queue and a fenced grid barrier, 2 barrier-free
                                dataflow loop, 3 level-synchronous loop with
                                block-local queues, 4 loop 1 with device-scope
-                               queue access and a fence-free barrier; all give
-                               identical results                              */
+                               queue access and a fence-free barrier, 5 loop 4
+                               with each wave's fanin records spread over its
+                               lanes and a counter barrier, its blocks on one
+                               XCD; all give identical results               */
   int step_times;           /* 1: record every expansion round's device time and
                                the paths it stored (cpg_step_times)           */
   int monotone_check;       /* 1: fail the query (error) when a child's binary32
@@ -178,7 +180,7 @@ int cpg_sfxt(cpg_handle* h);
  * memory (serving loops or harnesses that forbid hipMalloc): the SFXT level
  * loop (levelize, int-scaled relax, successor; the same dist/succ/level as
  * cpg_sfxt + cpg_get_sfxt) of a graph already on the device, with every
- * intermediate buffer in one static device array (about 24 MiB per device,
+ * intermediate buffer in one static device array (about 25 MiB per device,
  * held until the process ends), so the call allocates nothing.  Inputs are
  * device pointers: the fanout offsets (n+1), the fanin offsets (n+1), the
  * fanin records (2 ints per edge: fanin vertex, weight as float bits) and

@@ -345,10 +345,12 @@ def star_graph(fan=5000, seed=5):
     return n, np.array(u, np.int32), np.array(v, np.int32), w
 
 
-@pytest.mark.parametrize("method,blocks", [(2, 1), (2, 2), (2, 0), (3, 1), (3, 2), (3, 0)])
+@pytest.mark.parametrize("method,blocks", [(2, 1), (2, 2), (2, 0), (3, 1), (3, 2), (3, 0), (4, 1), (5, 1), (5, 2), (5, 0)])
 def test_sfxt_lds_queue_overflow(method, blocks):
-    """The dataflow loop's LDS pool and the local-queue loop's LDS level
-    buffer overflow into their global queues; results are unchanged."""
+    """The dataflow loop's LDS pool, the local-queue loop's LDS level
+    buffer and the level loops' LDS push queues overflow into their global
+    queues; results are unchanged.  The edge-parallel loop also takes the
+    3000-record vertex over many wave steps."""
     n, u, v, w = star_graph()
     say(f"case star graph sfxt method {method} blocks={blocks}")
     og = pyoracle.OracleGraph.from_edges(n, u, v, w)
@@ -366,12 +368,13 @@ def test_sfxt_lds_queue_overflow(method, blocks):
 
 @pytest.mark.parametrize("which", [1, 2, 3])
 def test_sfxt_methods_agree(which):
-    """The four SFXT loops (level loop with a global queue, dataflow loop,
-    local-queue level loop, fence-free level loop) give the oracle's
-    dist/succ/level on C1, C2 and the densified C3."""
+    """The five SFXT loops (level loop with a global queue, dataflow loop,
+    local-queue level loop, fence-free level loop, edge-parallel fence-free
+    level loop) give the oracle's dist/succ/level on C1, C2 and the
+    densified C3."""
     n, u, v, w, og, dist, succ, level, L = preset_graph(which)
     g = engine_for(n, u, v, w)
-    for method in (1, 2, 3, 4):
+    for method in (1, 2, 3, 4, 5):
         say(f"case C{which} sfxt method {method}")
         g.reset()
         g.report_paths(10, sfxt_method=method, kernel_timing=True)
@@ -391,7 +394,7 @@ def test_sfxt_dataflow_cycle_terminates():
     w = np.array([1, 2, 3, 4, 5], np.float32)
     say("case cycle graph")
     res = []
-    for method in (1, 2, 3, 4):
+    for method in (1, 2, 3, 4, 5):
         g = engine_for(5, u, v, w)
         g.report_paths(10, sfxt_method=method)
         res.append(g.sfxt() + (g.get_slacks(10),))

@@ -48,7 +48,7 @@ def c1():
     return n, u, v, w, og, dist, succ, level, L
 
 
-@pytest.mark.parametrize("method", [1, 2, 3, 4])
+@pytest.mark.parametrize("method", [1, 2, 3, 4, 5])
 def test_emulated_c1_query_equals_oracle(emu, c1, method):
     n, u, v, w, og, dist, succ, level, L = c1
     g = emu.CpGen()
@@ -79,11 +79,13 @@ def test_emulated_node_store_overflow_restarts_exactly(emu, c1):
         g.close()
 
 
-@pytest.mark.parametrize("method", [2, 3])
+@pytest.mark.parametrize("method", [2, 3, 5])
 def test_emulated_lds_queue_spill(emu, method):
     """One sink with 3000 fanins: settling it releases more vertices than a
     block's LDS queue holds (the dataflow loop's pool, the local-queue
-    loop's level buffer), so both spill to their global queues."""
+    loop's level buffer, the edge-parallel loop's level queue), so each
+    spills to its global queue; the edge-parallel loop also takes that
+    sink's 3000 records over many wave steps."""
     fan = 3000
     rng = np.random.default_rng(11)
     u, v = [], []
