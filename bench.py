@@ -415,9 +415,9 @@ def main():
     #  child 25 B (node slot)
     walk_bytes = 16.0 * st["hops"] + 16.0 * st["dev_visits"] + 12.0 * st["expanded"] \
         + 25.0 * (st["generated"] - st["roots"])
-    # the level loop's variant with 4 records per lane step runs where the
-    # graph has fewer than 4 edges per vertex, the 8-record one otherwise
-    lvl_kernel = "sfxt_levels_nf4_kernel" if m < 4 * n else "sfxt_levels_nf_kernel"
+    # the level loop's variant with 3 list positions per lane step runs
+    # where the graph has fewer than 4 edges per vertex, 8 otherwise
+    lvl_kernel = "sfxt_levels_e3_kernel" if m < 4 * n else "sfxt_levels_e8_kernel"
     kernels = {
         lvl_kernel: (levels_ms, levels_bytes, 1),
         "walk_kernel": (walk_ms, walk_bytes, max(walk_launches, 1)),

@@ -92,22 +92,35 @@ void pick_roots(const std::vector<int>& sd, bool quirk, float perc, std::vector<
                 int64_t& nroots, int64_t& dropped, float& minc) {
   const int nsrc = static_cast<int>(sd.size());
   kp.assign(std::max(1, nsrc), 0);
-  std::vector<float> rc;
-  rc.reserve(nsrc);
+  nroots = 0;
   for (int i = 0; i < nsrc; ++i)
     if (sd[i] != kIntMax) {
       kp[i] = 1;
-      rc.push_back(static_cast<float>(sd[i]) / static_cast<float>(kScaleUp));
+      ++nroots;
     }
-  nroots = static_cast<int64_t>(rc.size());
   dropped = 0;
   if (quirk && nroots > 0) {
-    // the (size_t)(p*n)-th smallest root cost: a selection, not a sort
-    std::vector<float> sel = rc;
+    // the (size_t)(p*n)-th smallest root cost (0-based): the largest of the
+    // at + 1 smallest distances, kept in a max-heap (at is 0.5% of the roots
+    // by default, so most distances are rejected against its top at once).
+    // The cost (float)d / SCALE_UP never reverses the order of distances.
     const float pn = perc * static_cast<float>(nroots);
-    const size_t at = std::min(static_cast<size_t>(pn), sel.size() - 1);
-    std::nth_element(sel.begin(), sel.begin() + static_cast<std::ptrdiff_t>(at), sel.end());
-    const float split = sel[at];
+    const size_t at = std::min(static_cast<size_t>(pn), static_cast<size_t>(nroots) - 1);
+    std::vector<int> heap;
+    heap.reserve(at + 1);
+    for (int i = 0; i < nsrc; ++i) {
+      const int x = sd[i];
+      if (x == kIntMax) continue;
+      if (heap.size() < at + 1) {
+        heap.push_back(x);
+        std::push_heap(heap.begin(), heap.end());
+      } else if (x < heap.front()) {
+        std::pop_heap(heap.begin(), heap.end());
+        heap.back() = x;
+        std::push_heap(heap.begin(), heap.end());
+      }
+    }
+    const float split = static_cast<float>(heap.front()) / static_cast<float>(kScaleUp);
     const int num_short = static_cast<int>(pn + 1);
     int kept_le = 0;
     for (int i = 0; i < nsrc; ++i) {
@@ -592,8 +605,14 @@ struct cpg_handle {
     const double tb0 = now_ms();
     const uint64_t nb = rc.size();
     launch_iota(batch.p, nb, grid, stream);
-    std::nth_element(rc.begin(), rc.begin() + static_cast<std::ptrdiff_t>(rc.size() / 10), rc.end());
-    const float p10 = rc[rc.size() / 10];
+    // the roots' 10th-percentile cost, which only sets the first trial's
+    // width: from an evenly strided sample of about 2048 roots
+    const size_t step = std::max<size_t>(1, rc.size() / 2048);
+    std::vector<float> sample;
+    sample.reserve(rc.size() / step + 1);
+    for (size_t i = 0; i < rc.size(); i += step) sample.push_back(rc[i]);
+    std::nth_element(sample.begin(), sample.begin() + static_cast<std::ptrdiff_t>(sample.size() / 10), sample.end());
+    const float p10 = sample[sample.size() / 10];
     if (trace) std::fprintf(stderr, "probe: host preparation %.3f ms\n", now_ms() - tb0);
     double gap = static_cast<double>(p10) - static_cast<double>(cmin);
     gap = std::max({gap, std::fabs(static_cast<double>(cmin)) * 0x1p-8,

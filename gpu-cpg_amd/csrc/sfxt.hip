@@ -468,7 +468,7 @@ __device__ void levels_edges_loop(const SfxtArgs& a) {
       if (i < le) {
         int u = ld_dev32(a.queue + i);
         unsigned spins = 0;
-        while (u < 0) {  // written before the barrier; a store still landing
+        while (u < 0) {  // written by a block after it arrived; still landing
           if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); break; }
           __builtin_amdgcn_s_sleep(1);
           u = ld_dev32(a.queue + i);

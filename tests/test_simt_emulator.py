@@ -66,6 +66,58 @@ def test_emulated_c1_query_equals_oracle(emu, c1, method):
         g.close()
 
 
+def test_emulated_relabelled_c1_equals_oracle(emu, c1):
+    """The level-order CSR relabelling: dist/succ/level (in original ids)
+    and the costs equal the oracle's, with and without an endpoint
+    subset."""
+    n, u, v, w, og, dist, succ, level, L = c1
+    g = emu.CpGen()
+    try:
+        g.load_edges(n, u, v, w)
+        g.set_csr_reorder(1)
+        g.report_paths(100)
+        gd, gs, gl = g.sfxt()
+        np.testing.assert_array_equal(gd, dist)
+        np.testing.assert_array_equal(gs, succ)
+        np.testing.assert_array_equal(gl, level)
+        want, _ = og.topk(dist, succ, 100)
+        assert bits_equal(want, g.get_slacks(100))
+        sinks = np.nonzero(np.bincount(u, minlength=n) == 0)[0]
+        m = np.zeros(n, np.uint8)
+        m[np.random.default_rng(5).choice(sinks, len(sinks) // 3, replace=False)] = 1
+        d2, s2, _, _ = og.sfxt(m)
+        want2, _ = og.topk(d2, s2, 100)
+        g.reset()
+        g.report_paths(100, endpoint_mask=m)
+        assert bits_equal(want2, g.get_slacks(100))
+    finally:
+        g.close()
+
+
+def test_emulated_batch_equals_oracle(emu):
+    """A batch of six small seeded designs (5K-9K vertices, K 112 to 10217):
+    the multi-query engine's rounds (the later ones replayed from a captured
+    HIP graph) give every design the oracle's best-first top K bit for
+    bit."""
+    picks = [12, 20, 24, 33, 36, 40]
+    b = emu.Batch()
+    want = []
+    try:
+        for i in picks:
+            n, u, v, w = emu.gen_preset(4, i)
+            k = int(emu.preset_k(4, i))
+            b.add(n, u, v, w, k)
+            og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+            dist, succ, _, _ = og.sfxt()
+            want.append((og.topk(dist, succ, k)[0], k))
+        b.run()
+        assert b.timings()["rounds"] > 1
+        for q, (wc, k) in enumerate(want):
+            assert bits_equal(wc, b.get_slacks(q, k)), f"design {picks[q]}"
+    finally:
+        b.close()
+
+
 def test_emulated_node_store_overflow_restarts_exactly(emu, c1):
     n, u, v, w, og, dist, succ, level, L = c1
     g = emu.CpGen()
