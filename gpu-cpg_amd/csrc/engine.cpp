@@ -406,14 +406,15 @@ struct cpg_handle {
   int to_orig(int v) const { return relabelled && v >= 0 ? h_orig[static_cast<size_t>(v)] : v; }
 
   // Level-loop grid: about one thread per vertex of an average level (known
-  // from the last levelization of this graph), within [16, 64] blocks; a
+  // from the last levelization of this graph), within [16, 128] blocks; a
   // grid barrier per level costs more with more blocks.
   static constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
+  static constexpr int64_t kOneXcdRecords = 50000;  // fanin records per level kept on one XCD
   int level_loop_blocks() const {
     if (sfxt_blocks > 0) return sfxt_blocks;
     if (last_diameter <= 0) return 64;
     const int64_t per_level = g.n / last_diameter;
-    return static_cast<int>(std::min<int64_t>(64, std::max<int64_t>(16, (per_level + 255) / 256)));
+    return static_cast<int>(std::min<int64_t>(128, std::max<int64_t>(16, (per_level + 255) / 256)));
   }
 
   // The endpoint subset of an n-byte mask as one flag per sink (in sink
@@ -476,14 +477,19 @@ struct cpg_handle {
         a.sink_mask = emask.p;
       }
       a.orig = relabelled ? orig_d.p : nullptr;
-      // 0 (automatic) is the edge-parallel fence-free level loop with its
-      // working blocks on one XCD (round-robin dispatch puts every 8th block
-      // on the same XCD)
+      // 0 (automatic) is the edge-parallel fence-free level loop
       const int method = sfxt_method == 0 ? 5 : sfxt_method;
+      // Its blocks on one XCD (round-robin dispatch puts every 8th block on
+      // the same XCD) while a level holds few fanin records, over all XCDs
+      // above that: on the 1.5M-vertex DAG (13.6K records per level) one
+      // XCD moves 459 MB per launch against 782 MB over all XCDs in the
+      // same time; the 30M-edge DAG (100K per level) runs 8.26 ms over all
+      // against 8.67 on one, the 256-design union graph 25 against 38 ms
+      const int64_t records_per_level = g.m / std::max(1, last_diameter);
+      a.place_stride = method == 5 && records_per_level <= kOneXcdRecords ? kXcds : 1;
       const bool dataflow = method == 2;
       if (dataflow && relabelled)
         throw std::runtime_error("the dataflow SFXT loop does not run on a relabelled graph");
-      a.place_stride = method == 5 ? kXcds : 1;
       int df_grid = 0;
       if (dataflow) {
         df_grid = sfxt_dataflow_grid(sfxt_blocks);

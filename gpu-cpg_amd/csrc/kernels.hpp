@@ -56,11 +56,11 @@ struct SfxtArgs {
   SfxtCtl* ctl;
   unsigned long long* dfw;  // dataflow loop: one progress word per block (dfw_n slots)
   int dfw_n;
-  // level loops: only blocks whose index is a multiple of place_stride take
-  // part (the others return at once); with a stride of 8 (the XCD count)
-  // every working block lands on the same XCD under round-robin dispatch
+  // edge-parallel level loop: only blocks whose index is a multiple of
+  // place_stride take part (the others return at once); with a stride of 8
+  // (the XCD count) every working block lands on one XCD under round-robin
+  // dispatch
   int place_stride;
-  int pad;
   int64_t m;  // edges (the fence-free loop picks its records per step by m / n)
 };
 

@@ -91,7 +91,6 @@ int sfxt_noalloc(int n, const int* oadjp, const int* iadjp, const int2* iedge, c
   a.dist = dist;
   a.succ = succ;
   a.level = level;
-  a.place_stride = 8;  // the working blocks on one XCD, as the default SFXT
   hipLaunchKernelGGL(noalloc_clear_ctl_kernel, dim3(1), dim3(64), 0, stream, a.ctl);
   CPG_HIP(hipGetLastError());
   launch_sfxt_dataflow_init(a, stream);  // counters, candidates, queue = sinks then -1

@@ -215,10 +215,8 @@ __device__ void levels_loop(const SfxtArgs& a) {
   __shared__ int s_qn;
   __shared__ unsigned s_base;
   SfxtCtl* ctl = a.ctl;
-  const unsigned stride = a.place_stride > 1 ? static_cast<unsigned>(a.place_stride) : 1u;
-  if (blockIdx.x % stride != 0) return;  // block-uniform: not a working block
-  const unsigned nblocks = gridDim.x / stride;
-  const int gtid = static_cast<int>(blockIdx.x / stride * blockDim.x + threadIdx.x);
+  const unsigned nblocks = gridDim.x;
+  const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   const int gthreads = static_cast<int>(nblocks * blockDim.x);
   if (threadIdx.x == 0) s_qn = 0;
   __syncthreads();
@@ -341,6 +339,108 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_kernel(SfxtArgs a) { l
 __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf_kernel(SfxtArgs a) { levels_loop<false, 8>(a); }
 __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf4_kernel(SfxtArgs a) { levels_loop<false, 4>(a); }
 
+// Counter barrier of the fence-free level loops (levels_edges_loop,
+// levels_multi_part): two 64-bit counters, ctl->lvl_bar[d & 1], each holding
+// the arrivals at its levels (top 24 bits) and the vertices pushed at them
+// (low 40 bits), both running totals that are never reset.  A block arrives
+// with one returning atomic that adds its LDS queue count; the value it
+// returns is also where the block's entries go: the level's entries before
+// it are the counter's pushes since this parity's previous level, so the
+// block writes its entries at le + (old - those) after arriving, while the
+// others are still on their way (the next level's readers wait on a slot
+// that is still -1).  A vertex pushed past the LDS queue reserves its slot
+// the same way, with a pushes-only add.  The waiters poll the counter itself
+// until every block has arrived; the last to arrive sees that in its own
+// atomic's result.  So a level ends after one returning atomic and, for the
+// others, the polls that see it, where the generation barrier also needs a
+// queue-tail atomic, a store-completion wait before arriving and the last
+// block's release store.
+//
+// Two counters make the running totals safe to read: a block that has seen
+// level d complete cannot be overtaken at level d + 2 (the same counter)
+// before it has itself arrived at level d + 1, so the value it reads is level
+// d's final one.  Arrival counts are compared modulo 2^24.
+constexpr unsigned long long kLevelLow = (1ull << 40) - 1;
+
+// Pushes counted on each parity's counter up to its previous level (two
+// scalars, not an indexed array, so that they stay in registers).
+struct LevelCounts {
+  unsigned long long last0 = 0ull, last1 = 0ull;
+  __device__ unsigned long long last(int d) const { return (d & 1) ? last1 : last0; }
+};
+
+// Queue slot for a vertex pushed at level d past the block's LDS queue.
+__device__ __forceinline__ unsigned counter_reserve(SfxtCtl* ctl, int d, unsigned le, const LevelCounts& lc) {
+  const unsigned long long old =
+      __hip_atomic_fetch_add(&ctl->lvl_bar[d & 1], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return le + static_cast<unsigned>((old & kLevelLow) - lc.last(d));
+}
+
+// Arrives at level d's barrier with the block's LDS queue s_q[0, min(*s_qn,
+// kLdsQueue)), writes those entries, waits for the other nblocks - 1 blocks
+// and clears *s_qn.  Returns false on error, else the next level's end.
+__device__ __forceinline__ bool counter_barrier(SfxtCtl* ctl, int* queue, const int* s_q, int* s_qn,
+                                                unsigned nblocks, int d, unsigned le, LevelCounts& lc,
+                                                unsigned* next_end) {
+  __shared__ unsigned s_base;
+  __shared__ unsigned long long s_low;
+  __shared__ int s_ok;
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's atomics and stores are done
+  __syncthreads();
+  const int cnt = min(*s_qn, kLdsQueue);
+  const int par = d & 1;
+  const unsigned long long last = lc.last(d);
+  unsigned long long mine = 0;
+  if (threadIdx.x == 0) {
+    const unsigned long long add = (1ull << 40) | static_cast<unsigned long long>(cnt);
+    const unsigned long long old =
+        __hip_atomic_fetch_add(&ctl->lvl_bar[par], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_base = le + static_cast<unsigned>((old & kLevelLow) - last);
+    mine = old + add;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) st_dev32(queue + s_base + i, s_q[i]);
+  if (threadIdx.x == 0) {
+    *s_qn = 0;  // every thread read the count before the barrier above
+    const unsigned target = ((static_cast<unsigned>(d) >> 1) + 1u) * nblocks;
+    unsigned long long w = mine;
+    int ok = 1;
+    unsigned spins = 0;
+    // arrivals so far minus the target, as a signed 24-bit difference
+    auto short_of = [&](unsigned long long v) {
+      const unsigned diff = (static_cast<unsigned>(v >> 40) - target) & 0xffffffu;
+      return (diff & 0x800000u) != 0;
+    };
+    while (short_of(w)) {
+      if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
+      if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); ok = 0; break; }
+      __builtin_amdgcn_s_sleep(1);
+      w = ld_dev64(&ctl->lvl_bar[par]);
+    }
+    s_low = w & kLevelLow;
+    s_ok = ok;
+  }
+  __syncthreads();
+  const unsigned long long low = s_low;
+  *next_end = le + static_cast<unsigned>(low - last);
+  if (par) lc.last1 = low;
+  else lc.last0 = low;
+  return s_ok != 0;
+}
+
+// Queue entry i of the fence-free loops: written by a block after it
+// arrived at the previous level's barrier, so it may still be landing (-1).
+__device__ __forceinline__ int queue_entry(const SfxtArgs& a, unsigned i) {
+  int u = ld_dev32(a.queue + i);
+  unsigned spins = 0;
+  while (u < 0) {
+    if (++spins > kSpinLimit) { atomicExch(&a.ctl->err, 1); break; }
+    __builtin_amdgcn_s_sleep(1);
+    u = ld_dev32(a.queue + i);
+  }
+  return u;
+}
+
 // Edge-parallel fence-free level loop: the same peel, relax and successor
 // as levels_loop<false>, with the fanin release spread over the records of
 // a wave's vertices instead of over one vertex per lane.
@@ -355,36 +455,12 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_nf4_kernel(SfxtArgs a)
 // vertex's record count rounded up to its step) were 40% of that loop's
 // atomics on the 1.5M-vertex DAG; here only the positions past the wave's
 // last record in its final step issue no-ops (on the pad slots).  Records of
-// one vertex sit in consecutive lanes, so their loads coalesce.
-//
-// Level barrier: two 64-bit counters, ctl->lvl_bar[d & 1], each holding the
-// arrivals at its levels (top 24 bits) and the vertices pushed at them (low
-// 40 bits), both running totals that are never reset.  A block arrives with
-// one returning atomic that adds its LDS queue count; the value it returns
-// is also where the block's entries go: the level's entries before it are
-// the counter's pushes since this parity's previous level (last_low), so the
-// block writes its entries at le + (old - last_low) after arriving, while
-// the others are still on their way (the next level's readers wait on a
-// slot that is still -1).  A vertex pushed past the LDS queue reserves its
-// slot the same way, with a pushes-only add.  The waiters poll the counter
-// itself until every working block has arrived; the last to arrive sees that
-// in its own atomic's result.  So a level ends after one returning atomic
-// and, for the others, the polls that see it, where the generation barrier
-// also needed a queue-tail atomic, a store-completion wait before arriving
-// and the last block's release store.
-//
-// Two counters make the running totals safe to read: a block that has seen
-// level d complete cannot be overtaken at level d + 2 (the same counter)
-// before it has itself arrived at level d + 1, so the value it reads is level
-// d's final one.  Arrival counts are compared modulo 2^24.
+// one vertex sit in consecutive lanes, so their loads coalesce.  Each level
+// ends at the counter barrier.
 template <int kS>
 __device__ void levels_edges_loop(const SfxtArgs& a) {
   __shared__ int s_q[kLdsQueue];
   __shared__ int s_qn;
-  __shared__ unsigned s_base;
-  __shared__ unsigned long long s_low;
-  __shared__ int s_ok;
-  constexpr unsigned long long kLow = (1ull << 40) - 1;
   SfxtCtl* ctl = a.ctl;
   const unsigned stride = a.place_stride > 1 ? static_cast<unsigned>(a.place_stride) : 1u;
   if (blockIdx.x % stride != 0) return;  // block-uniform: not a working block
@@ -398,64 +474,11 @@ __device__ void levels_edges_loop(const SfxtArgs& a) {
 
   unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
   int d = 0;
-  // pushes counted on each parity's counter up to its previous level (two
-  // scalars, not an indexed array, so that they stay in registers)
-  unsigned long long last_low0 = 0ull, last_low1 = 0ull;
+  LevelCounts lc;
   auto push = [&](int x) {
     const int p = atomicAdd(&s_qn, 1);
-    if (p < kLdsQueue) {
-      s_q[p] = x;
-    } else {  // past the LDS queue: reserve a slot of this level directly
-      const unsigned long long old =
-          __hip_atomic_fetch_add(&ctl->lvl_bar[d & 1], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned long long last = (d & 1) ? last_low1 : last_low0;
-      st_dev32(a.queue + le + static_cast<unsigned>((old & kLow) - last), x);
-    }
-  };
-  // Arrives at level d's barrier, writes the block's entries and waits for
-  // the other blocks; returns false on error, else the next level's end.
-  auto level_barrier = [&](unsigned* next_end) -> bool {
-    __builtin_amdgcn_s_waitcnt(0);  // this thread's atomics and stores are done
-    __syncthreads();
-    const int cnt = min(s_qn, kLdsQueue);
-    const int par = d & 1;
-    const unsigned long long last = par ? last_low1 : last_low0;
-    unsigned long long mine = 0;
-    if (threadIdx.x == 0) {
-      const unsigned long long add = (1ull << 40) | static_cast<unsigned long long>(cnt);
-      const unsigned long long old =
-          __hip_atomic_fetch_add(&ctl->lvl_bar[par], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_base = le + static_cast<unsigned>((old & kLow) - last);
-      mine = old + add;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) st_dev32(a.queue + s_base + i, s_q[i]);
-    if (threadIdx.x == 0) {
-      s_qn = 0;  // every thread read the count before the barrier above
-      const unsigned target = ((static_cast<unsigned>(d) >> 1) + 1u) * nblocks;
-      unsigned long long w = mine;
-      int ok = 1;
-      unsigned spins = 0;
-      // arrivals so far minus the target, as a signed 24-bit difference
-      auto short_of = [&](unsigned long long v) {
-        const unsigned diff = (static_cast<unsigned>(v >> 40) - target) & 0xffffffu;
-        return (diff & 0x800000u) != 0;
-      };
-      while (short_of(w)) {
-        if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
-        if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); ok = 0; break; }
-        __builtin_amdgcn_s_sleep(1);
-        w = ld_dev64(&ctl->lvl_bar[par]);
-      }
-      s_low = w & kLow;
-      s_ok = ok;
-    }
-    __syncthreads();
-    const unsigned long long low = s_low;
-    *next_end = le + static_cast<unsigned>(low - last);
-    if (par) last_low1 = low;
-    else last_low0 = low;
-    return s_ok != 0;
+    if (p < kLdsQueue) s_q[p] = x;
+    else st_dev32(a.queue + counter_reserve(ctl, d, le, lc), x);  // past the LDS queue
   };
 
   float local_max = 0.0f;
@@ -466,13 +489,7 @@ __device__ void levels_edges_loop(const SfxtArgs& a) {
       const unsigned i = i0 + static_cast<unsigned>(lane);
       int b = 0, deg = 0, du = kIntMax, tie = 0;
       if (i < le) {
-        int u = ld_dev32(a.queue + i);
-        unsigned spins = 0;
-        while (u < 0) {  // written by a block after it arrived; still landing
-          if (++spins > kSpinLimit) { atomicExch(&ctl->err, 1); break; }
-          __builtin_amdgcn_s_sleep(1);
-          u = ld_dev32(a.queue + i);
-        }
+        const int u = queue_entry(a, i);
         if (u >= 0) {
           const unsigned long long key = ld_dev64(a.best + u);
           b = a.iadjp[u];
@@ -542,7 +559,7 @@ __device__ void levels_edges_loop(const SfxtArgs& a) {
       }
     }
     unsigned ne = 0;
-    if (!level_barrier(&ne)) return;
+    if (!counter_barrier(ctl, a.queue, s_q, &s_qn, nblocks, d, le, lc, &ne)) return;
     if (ne == le) break;
     lb = le;
     le = ne;
@@ -1129,7 +1146,8 @@ __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMu
       m.distf[q * n + v] = static_cast<float>(kIntMax) / static_cast<float>(kScaleUp);
     }
   }
-  for (int i = gtid; i < a.nsinks; i += gthreads) a.queue[i] = a.sinks[i];
+  // the queue: the sinks, then -1 until the level loop writes a slot
+  for (int v = gtid; v < a.n; v += gthreads) a.queue[v] = v < a.nsinks ? a.sinks[v] : -1;
   for (int q = gtid; q < m.q; q += gthreads) m.maxd[q] = 0u;
   if (gtid == 0) a.ctl->qtail = static_cast<unsigned>(a.nsinks);
 }
@@ -1142,14 +1160,16 @@ __global__ void __launch_bounds__(256) sfxt_init_multi_kernel(SfxtArgs a, SfxtMu
 // vertex-major distances dv[w*Q + q] of the settled fanouts, and a
 // shuffle reduction over each subset's lanes gives the minimum and the
 // largest fanout id at it.  No per-subset atomics: the only atomics are
-// the fanin releases, shared by the subsets.  The per-level grid barrier
-// is shared by all Q subsets, so Q subsets cost about one loop.
+// the fanin releases, shared by the subsets.  The per-level barrier is
+// shared by all Q subsets, so Q subsets cost about one loop.  Fence-free:
+// queue entries and dv[] are written and read at device scope, and each
+// level ends at the counter barrier (the queue must hold the sinks and -1
+// beyond, as sfxt_init_multi leaves it).
 // It runs on the blocks [b0, b0 + nblk) of a launch that holds the level
 // loops of several graphs (one partition each, with its own barrier).
 __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigned bid, unsigned nblk) {
   __shared__ int s_q[kLdsQueue];
   __shared__ int s_qn;
-  __shared__ unsigned s_base;
   SfxtCtl* ctl = a.ctl;
   const unsigned nblocks = nblk;
   const int lane = __lane_id();
@@ -1163,32 +1183,21 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
   if (threadIdx.x == 0) s_qn = 0;
   __syncthreads();
 
+  unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
+  int d = 0;
+  LevelCounts lc;
   auto push = [&](int x) {
     const int p = atomicAdd(&s_qn, 1);
     if (p < kLdsQueue) s_q[p] = x;
-    else a.queue[atomicAdd(&ctl->qtail, 1u)] = x;
-  };
-  // Writes the block's LDS queue out; returns every vertex the block
-  // pushed this level (those that spilled past the LDS queue included).
-  auto flush = [&]() -> unsigned {
-    __syncthreads();
-    const int total = s_qn;
-    const int cnt = min(total, kLdsQueue);
-    if (threadIdx.x == 0) s_base = cnt ? atomicAdd(&ctl->qtail, static_cast<unsigned>(cnt)) : 0u;
-    __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) a.queue[s_base + i] = s_q[i];
-    __syncthreads();
-    if (threadIdx.x == 0) s_qn = 0;
-    return static_cast<unsigned>(total);
+    else st_dev32(a.queue + counter_reserve(ctl, d, le, lc), x);  // past the LDS queue
   };
 
-  unsigned lb = 0, le = static_cast<unsigned>(a.nsinks);
-  int d = 0;
   float local_max = 0.0f;  // lanes < Q: subset lane's largest finite |distf|
   while (true) {
     if (bid == 0 && threadIdx.x == 0) a.lvlp[d] = static_cast<int>(lb);
     for (unsigned i = lb + static_cast<unsigned>(wave); i < le; i += static_cast<unsigned>(nwaves)) {
-      const int u = a.queue[i];
+      const int u = queue_entry(a, i);
+      if (u < 0) continue;  // (only after a spin-limit error; the barrier ends the loop)
       const int ob = a.oadjp[u], oe = a.oadjp[u + 1];
       const int ib = a.iadjp[u], ie = a.iadjp[u + 1];
       int du = kIntMax, s = -1;
@@ -1210,7 +1219,7 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
           }
 #pragma unroll
           for (int t = 0; t < kMultiSteps; ++t)
-            dw[t] = w[t] >= 0 ? m.dv[static_cast<size_t>(w[t]) * Q + sub] : kIntMax;
+            dw[t] = w[t] >= 0 ? ld_dev32(m.dv + static_cast<size_t>(w[t]) * Q + sub) : kIntMax;
 #pragma unroll
           for (int t = 0; t < kMultiSteps; ++t) {
             if (dw[t] == kIntMax) continue;
@@ -1239,7 +1248,7 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
         m.dist[lane * n + u] = du;
         m.succ[lane * n + u] = s;
         m.distf[lane * n + u] = df;
-        m.dv[static_cast<size_t>(u) * Q + lane] = du;
+        st_dev32(m.dv + static_cast<size_t>(u) * Q + lane, du);
         if (du != kIntMax) local_max = fmaxf(local_max, fabsf(df));
       }
       if (lane == 0) a.level[u] = d;
@@ -1276,9 +1285,8 @@ __device__ void levels_multi_part(const SfxtArgs& a, const SfxtMulti& m, unsigne
           if (j0 + t * 64 + lane < ie && old[t] == 1) push(x[t]);
       }
     }
-    const unsigned pushed = flush();
     unsigned ne = 0;
-    if (!grid_barrier(ctl, nblocks, pushed, le, &ne)) return;
+    if (!counter_barrier(ctl, a.queue, s_q, &s_qn, nblocks, d, le, lc, &ne)) return;
     if (ne == le) break;
     lb = le;
     le = ne;
@@ -1623,38 +1631,33 @@ void launch_sfxt_init(const SfxtArgs& args, hipStream_t stream) {
   CPG_HIP(hipGetLastError());
 }
 
-// Working blocks of a level loop and its launch grid (place_stride blocks
-// per working block).
-static int placed_grid(int blocks, int resident, int stride) {
-  const int working = std::max(1, std::min(blocks, resident / std::max(1, stride)));
-  return working * std::max(1, stride);
-}
+// Level-loop grid: the blocks asked for, within the resident capacity.
+static int loop_grid(int blocks, int resident) { return std::max(1, std::min(blocks, resident)); }
 
 int launch_sfxt_levels(const SfxtArgs& args, int blocks, hipStream_t stream) {
-  const int stride = std::max(1, args.place_stride);
-  const int grid = placed_grid(blocks, resident_blocks(sfxt_levels_kernel, kSfxtBlock), stride);
+  const int grid = loop_grid(blocks, resident_blocks(sfxt_levels_kernel, kSfxtBlock));
   launch_resident(sfxt_levels_kernel, grid, kSfxtBlock, args, stream);
-  return grid / stride;
+  return grid;
 }
 
 int launch_sfxt_levels_nf(const SfxtArgs& args, int blocks, hipStream_t stream) {
-  const int stride = std::max(1, args.place_stride);
   // mean in-degree m / n: under 4 -> 4 records per step, else 8
   const int64_t m = args.m, n = std::max(1, args.n);
   auto k = m < 4 * n ? sfxt_levels_nf4_kernel : sfxt_levels_nf_kernel;
-  const int grid = placed_grid(blocks, resident_blocks(k, kSfxtBlock), stride);
+  const int grid = loop_grid(blocks, resident_blocks(k, kSfxtBlock));
   launch_resident(k, grid, kSfxtBlock, args, stream);
-  return grid / stride;
+  return grid;
 }
 
 int launch_sfxt_levels_edges(const SfxtArgs& args, int blocks, hipStream_t stream) {
-  const int stride = std::max(1, args.place_stride);
   // mean in-degree m / n: under 4 -> 3 list positions per lane step, else 8
   const int64_t m = args.m, n = std::max(1, args.n);
   auto k = m < 4 * n ? sfxt_levels_e3_kernel : sfxt_levels_e8_kernel;
-  const int grid = placed_grid(blocks, resident_blocks(k, kSfxtBlock), stride);
-  launch_resident(k, grid, kSfxtBlock, args, stream);
-  return grid / stride;
+  // place_stride launch blocks per working block (the others return at once)
+  const int stride = std::max(1, args.place_stride);
+  const int working = loop_grid(blocks, resident_blocks(k, kSfxtBlock) / stride);
+  launch_resident(k, working * stride, kSfxtBlock, args, stream);
+  return working;
 }
 
 int sfxt_dataflow_grid(int blocks) {

@@ -66,8 +66,8 @@ typedef struct {
                                block-local queues, 4 loop 1 with device-scope
                                queue access and a fence-free barrier, 5 loop 4
                                with each wave's fanin records spread over its
-                               lanes and a counter barrier, its blocks on one
-                               XCD; all give identical results               */
+                               lanes and a counter barrier; all give
+                               identical results                              */
   int step_times;           /* 1: record every expansion round's device time and
                                the paths it stored (cpg_step_times)           */
   int monotone_check;       /* 1: fail the query (error) when a child's binary32
