@@ -73,6 +73,31 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Pinned host memory (hipHostMalloc) for small transfers on the query path:
+// copies from and to it go straight to the DMA engine, with no staging.
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void reserve(size_t want) {
+    if (want <= n) return;
+    release();
+    void* q = nullptr;
+    CPG_HIP(hipHostMalloc(&q, sizeof(T) * want, hipHostMallocDefault));
+    p = static_cast<T*>(q);
+    n = want;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+};
+
 template <class T>
 void h2d(T* d, const T* h, size_t n, hipStream_t s) {
   if (n) CPG_HIP(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
@@ -170,6 +195,9 @@ struct cpg_handle {
 
   // SFXT
   DevBuf<int> deps, queue, lvlp, level, dist, succ, src_dist, big;
+  PinnedBuf<int> h_src_dist;      // the sources' distances, copied back per query
+  PinnedBuf<uint8_t> h_keep;      // the roots' keep flags, copied up per query
+  PinnedBuf<SfxtCtl> h_sctl;      // the SFXT control block, copied back per query
   DevBuf<unsigned long long> best;
   DevBuf<float> distf;
   DevBuf<int4> vrec, devs, devs_tmp;
@@ -435,16 +463,32 @@ struct cpg_handle {
   // Runs the SFXT for the endpoint subset of mask (n bytes, null = every
   // sink) unless a cached SFXT for that subset is valid.  Returns elapsed ms.
   double run_sfxt(bool count_cache, const uint8_t* mask = nullptr) {
+    const double t0 = now_ms();
+    if (!sfxt_enqueue(count_cache, mask)) return 0.0;
+    SfxtCtl c{};
+    d2h(&c, sctl.p, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    sfxt_finish(c, count_cache, mask);
+    return now_ms() - t0;
+  }
+
+  // The SFXT in two halves, so that a caller can put its own work on the
+  // stream before the one synchronisation: sfxt_enqueue puts the kernels on
+  // the stream (false: the cached SFXT stands, nothing was queued), and
+  // sfxt_finish checks the control block the caller copied back after them.
+  std::vector<uint8_t> pending_sinks;  // endpoint subset of the queued SFXT
+  int pending_lift = 0;                // lifting levels its index was built with
+
+  bool sfxt_enqueue(bool count_cache, const uint8_t* mask) {
     upload_graph();
     std::vector<uint8_t> subset = sink_subset(mask);
     if (sfxt_valid && cache_enabled && subset == sfxt_sinks) {
       if (count_cache) ++cache_hits;
-      return 0.0;
+      return false;
     }
     hb.valid = false;
     sfxt_valid = false;
     if (cache_enabled && count_cache) ++cache_misses;
-    const double t0 = now_ms();
     const size_t n = static_cast<size_t>(g.n), m = static_cast<size_t>(g.m);
     // best[] and deps[] carry the level loops' pad slots for no-op atomics
     deps.reserve(n + kSfxtPad); queue.reserve(n); lvlp.reserve(n + 2); level.reserve(n);
@@ -513,22 +557,42 @@ struct cpg_handle {
                                          : launch_sfxt_levels(a, level_loop_blocks(), stream);
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_levels.b, stream));
       if (relabelled) launch_sfxt_succ_relabel(succ.p, perm_d.p, g.n, stream);
-      d2h(&c, sctl.p, 1, stream);
-      CPG_HIP(hipStreamSynchronize(stream));
-      if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
-      const int lv = sfxt_lift_levels(c.num_levels);
+      // The lifting table needs the level count.  Levels depend on the
+      // graph alone (not on the endpoint subset), so after this graph's
+      // first levelization the count is known without waiting for the loop.
+      int lv = 0;
+      if (last_diameter > 0) {
+        lv = sfxt_lift_levels(last_diameter);
+      } else {
+        d2h(&c, sctl.p, 1, stream);
+        CPG_HIP(hipStreamSynchronize(stream));
+        if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
+        lv = sfxt_lift_levels(c.num_levels);
+      }
       lift.reserve(static_cast<size_t>(lv) * n);
       a.lift = lift.p;
       launch_sfxt_index(a, lv, g.m, stream);
+      pending_lift = lv;
       if (kernel_timing) CPG_HIP(hipEventRecord(ev_sfxt.b, stream));
     }
-    d2h(&c, sctl.p, 1, stream);
-    CPG_HIP(hipStreamSynchronize(stream));
-    if (kernel_timing && n > 0) {
+    pending_sinks = std::move(subset);
+    return true;
+  }
+
+  void sfxt_finish(const SfxtCtl& c, bool count_cache, const uint8_t* mask) {
+    if (kernel_timing && g.n > 0) {
       timing.sfxt_kernel_ms = ev_sfxt.ms();
       timing.sfxt_levels_ms = ev_levels.ms();
     }
     if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
+    if (g.n > 0 && sfxt_lift_levels(c.num_levels) > pending_lift) {
+      // more levels than the table was sized for (not seen on one graph,
+      // whose levels do not change): run it again with the count known
+      if (last_diameter == c.num_levels) throw std::runtime_error("SFXT lifting table sizing failed");
+      last_diameter = c.num_levels;
+      run_sfxt(count_cache, mask);
+      return;
+    }
     if (c.nsv_err == 2) throw std::runtime_error("SFXT long deviation list without sort buffer");
     if (c.nsv_err) throw std::runtime_error("SFXT next-smaller descent found no answer");
     diameter = c.num_levels;
@@ -536,9 +600,8 @@ struct cpg_handle {
     uint32_t bits = c.max_abs_distf_bits;
     std::memcpy(&max_abs_distf, &bits, 4);
     sfxt_valid = true;
-    sfxt_sinks = std::move(subset);
+    sfxt_sinks = std::move(pending_sinks);
     h_succ.clear();
-    return now_ms() - t0;
   }
 
   // Device bytes per node-store slot: cost, to, from, parent, level, key (4
@@ -701,21 +764,40 @@ struct cpg_handle {
     sfxt_blocks = o.sfxt_blocks;
     sfxt_method = o.sfxt_method;
     const double tq0 = now_ms();
-    timing.sfxt_ms = run_sfxt(true, o.endpoint_mask);
+    const bool trace = std::getenv("CPG_TRACE") != nullptr;
+    // The SFXT and the sources' distances (roots: sources with finite dist,
+    // gpucpg/gpucpg.cu:18502-18513) come back in one synchronisation;
+    // sfxt_ms covers both.
+    const int nsrc = static_cast<int>(g.srcs.size());
+    src_dist.reserve(std::max(1, nsrc));
+    h_src_dist.reserve(std::max(1, nsrc));
+    h_sctl.reserve(1);
+    auto copy_source_dists = [&]() {
+      if (g.n > 0 && nsrc > 0) {
+        launch_gather_int(dist.p, srcs.p, nsrc, src_dist.p, stream);
+        d2h(h_src_dist.p, src_dist.p, nsrc, stream);
+      }
+    };
+    const bool queued = sfxt_enqueue(true, o.endpoint_mask);
+    copy_source_dists();
+    if (queued) d2h(h_sctl.p, sctl.p, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    if (queued) {
+      const int lift_before = pending_lift;
+      sfxt_finish(*h_sctl.p, true, o.endpoint_mask);
+      if (pending_lift != lift_before) {  // it ran the SFXT again (a short lifting table)
+        copy_source_dists();
+        CPG_HIP(hipStreamSynchronize(stream));
+      }
+    }
+    timing.sfxt_ms = now_ms() - tq0;
     if (k <= 0 || g.n == 0) {
       timing.query_ms = now_ms() - tq0;
       have_result = true;
       return;
     }
     const double tp0 = now_ms();
-    const bool trace = std::getenv("CPG_TRACE") != nullptr;
-    // roots: sources with finite dist (gpucpg/gpucpg.cu:18502-18513)
-    const int nsrc = static_cast<int>(g.srcs.size());
-    std::vector<int> sd(nsrc);
-    src_dist.reserve(std::max(1, nsrc));
-    launch_gather_int(dist.p, srcs.p, nsrc, src_dist.p, stream);
-    d2h(sd.data(), src_dist.p, nsrc, stream);
-    CPG_HIP(hipStreamSynchronize(stream));
+    std::vector<int> sd(h_src_dist.p, h_src_dist.p + nsrc);
     const double tr_copy = now_ms();
     std::vector<uint8_t> kp;
     int64_t nroots = 0, dropped = 0;
@@ -727,7 +809,9 @@ struct cpg_handle {
     timing.roots = nroots - dropped;
     timing.roots_dropped = dropped;
     keep.reserve(std::max(1, nsrc));
-    h2d(keep.p, kp.data(), nsrc, stream);
+    h_keep.reserve(std::max(1, nsrc));
+    std::copy(kp.begin(), kp.begin() + nsrc, h_keep.p);  // the previous query's copy has completed
+    h2d(keep.p, h_keep.p, nsrc, stream);
 
     const uint64_t batch_target = o.batch > 0 ? static_cast<uint64_t>(o.batch) : 131072ull;
     uint64_t cap = o.max_nodes > 0 ? static_cast<uint64_t>(o.max_nodes)

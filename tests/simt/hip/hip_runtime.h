@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <tuple>
 #include <type_traits>
@@ -223,6 +224,16 @@ hipError_t hipMalloc(T** p, size_t bytes) {
   return e;
 }
 hipError_t hipFree(void* p);
+// Pinned host memory is ordinary host memory here.
+constexpr unsigned hipHostMallocDefault = 0;
+inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
+  *p = std::malloc(bytes ? bytes : 1);
+  return *p ? hipSuccess : 2;
+}
+inline hipError_t hipHostFree(void* p) {
+  std::free(p);
+  return hipSuccess;
+}
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind k, hipStream_t s);
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
   *p = const_cast<void*>(sym);
