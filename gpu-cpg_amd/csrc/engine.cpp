@@ -198,6 +198,16 @@ struct cpg_handle {
   PinnedBuf<int> h_src_dist;      // the sources' distances, copied back per query
   PinnedBuf<uint8_t> h_keep;      // the roots' keep flags, copied up per query
   PinnedBuf<SfxtCtl> h_sctl;      // the SFXT control block, copied back per query
+  PinnedBuf<PfxtCtl> h_pctl;      // the PFXT control block, copied back per round
+  PinnedBuf<unsigned long long> h_nsel;  // the final selection's path count
+
+  // The PFXT control block after the work on the stream so far.
+  PfxtCtl read_pctl() {
+    h_pctl.reserve(1);
+    d2h(h_pctl.p, pctl.p, 1, stream);
+    CPG_HIP(hipStreamSynchronize(stream));
+    return *h_pctl.p;
+  }
   DevBuf<unsigned long long> best;
   DevBuf<float> distf;
   DevBuf<int4> vrec, devs, devs_tmp;
@@ -695,9 +705,7 @@ struct cpg_handle {
       const uint32_t uk = float_key(u0);
       launch_probe_setup(pctl.p, uk, window_shift(klo, uk), nb, nb, stream);
       launch_walk(wa, true, grid, stream);
-      PfxtCtl pc{};
-      d2h(&pc, pctl.p, 1, stream);
-      CPG_HIP(hipStreamSynchronize(stream));
+      PfxtCtl pc = read_pctl();
       if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
       if (pc.overflow) {  // the trial bound's children do not fit the store
         probe_overflowed = true;
@@ -718,8 +726,7 @@ struct cpg_handle {
         CPG_HIP(hipMemsetAsync(n_flag.p, 1, nb, stream));
         hist_nodes(0, false, false);
         launch_probe_finish(pctl.p, hist_all.p, static_cast<unsigned long long>(k), nb, stream);
-        d2h(&pc, pctl.p, 1, stream);
-        CPG_HIP(hipStreamSynchronize(stream));
+        pc = read_pctl();
         timing.probes = t + 1;
         return pc.ukey;
       }
@@ -933,8 +940,7 @@ struct cpg_handle {
             CPG_HIP(hipEventRecord(round_event(static_cast<size_t>(r) + 1), stream));
           }
         }
-        d2h(&c, pctl.p, 1, stream);
-        CPG_HIP(hipStreamSynchronize(stream));
+        c = read_pctl();
         if (kernel_timing) {
           const double cm = ev_count.ms(), fm = ev_fill.ms();
           timing.walk_count_ms += cm;
@@ -995,9 +1001,10 @@ struct cpg_handle {
     CPG_HIP(hipMemsetAsync(nsel.p, 0, sizeof(unsigned long long), stream));
     const NodeStore ns = node_store();
     launch_build_sort_keys(ns, pctl.p, nn, skey[0].p, sval[0].p, nsel.p, stream);
-    unsigned long long nl = 0;
-    d2h(&nl, nsel.p, 1, stream);
+    h_nsel.reserve(1);
+    d2h(h_nsel.p, nsel.p, 1, stream);
     CPG_HIP(hipStreamSynchronize(stream));
+    const unsigned long long nl = *h_nsel.p;
     ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
     SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
     const int bytes[6] = {0, 1, 4, 5, 6, 7};
@@ -1813,9 +1820,7 @@ struct cpg_server {
       h.keep.reserve(std::max(1, nsrc));
       h2d(h.keep.p, kp.data(), nsrc, h.stream);
       launch_roots(h.srcs.p, h.keep.p, nsrc, aq.dist, aq.distf, h.node_store(), h.pctl.p, h.stream);
-      PfxtCtl pc{};
-      d2h(&pc, h.pctl.p, 1, h.stream);
-      CPG_HIP(hipStreamSynchronize(h.stream));
+      PfxtCtl pc = h.read_pctl();
       u0 = float_key(rc[q.k - 1]);
       q.seed_n = pc.node_n;
       std::lock_guard<std::mutex> lk(stats_mu);
@@ -1849,9 +1854,7 @@ struct cpg_server {
           std::lock_guard<std::mutex> lk(stats_mu);
           stats.probe_ms += now_ms() - tp;
         }
-        PfxtCtl pc{};
-        d2h(&pc, h.pctl.p, 1, h.stream);
-        CPG_HIP(hipStreamSynchronize(h.stream));
+        PfxtCtl pc = h.read_pctl();
         if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
         q.seed_n = std::min<uint64_t>(pc.node_n, cap);
         if (!(u0 == kKeyInf && h.probe_overflowed) || attempt >= 4 || cap >= 0x7fffffffull) break;
