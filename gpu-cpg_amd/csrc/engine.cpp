@@ -1005,7 +1005,7 @@ struct cpg_handle {
     d2h(h_nsel.p, nsel.p, 1, stream);
     CPG_HIP(hipStreamSynchronize(stream));
     const unsigned long long nl = *h_nsel.p;
-    ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
+    ghist.reserve(sort_ghist_len(nl));
     SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
     const int bytes[6] = {0, 1, 4, 5, 6, 7};
     const int res = radix_sort_pairs(sb, nl, bytes, 6, stream);
@@ -1512,7 +1512,7 @@ struct cpg_batch {
     d2h(&nl, nsel.p, 1, stream);
     d2h(pq.data(), per_q.p, nqc, stream);
     CPG_HIP(hipStreamSynchronize(stream));
-    ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
+    ghist.reserve(sort_ghist_len(nl));
     SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
     std::vector<int> bytes = {0, 1, 2, 3, 4, 5};
     if (nqc > 1) bytes.push_back(6);
@@ -2312,7 +2312,7 @@ struct cpg_server {
     d2h(&nl, nsel.p, 1, stream);
     d2h(pq.data(), per_q.p, nslots, stream);
     CPG_HIP(hipStreamSynchronize(stream));
-    ghist.reserve(256 * std::max<uint64_t>(1, sort_tiles(nl)));
+    ghist.reserve(sort_ghist_len(nl));
     SortBufs sb{{skey[0].p, skey[1].p}, {sval[0].p, sval[1].p}, ghist.p, nl};
     const int bytes[7] = {0, 1, 2, 3, 4, 5, 6};  // slot < 256: bits 48..55
     const int res = radix_sort_pairs(sb, nl, bytes, 7, stream);

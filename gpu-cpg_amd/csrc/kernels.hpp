@@ -318,11 +318,12 @@ void launch_bvss_discover(const BvssArgs& a, const int* fptr, const int* fsrc, i
 struct SortBufs {
   uint64_t* key[2];
   uint32_t* val[2];
-  unsigned* ghist;        // 256 * max tiles
+  unsigned* ghist;        // sort_ghist_len(n) counts: 256 digits x tiles, then 256 digit totals
   uint64_t cap;
 };
 int radix_sort_pairs(SortBufs& b, uint64_t n, const int* bytes, int nbytes, hipStream_t s);
 uint64_t sort_tiles(uint64_t n);
+uint64_t sort_ghist_len(uint64_t n);
 
 // Sort input: stored paths with a nonzero key <= U (nmax = stored count).
 void launch_build_sort_keys(const NodeStore& nodes, const PfxtCtl* ctl, uint64_t nmax,

@@ -80,6 +80,56 @@ __global__ void __launch_bounds__(kScanBlock) sort_scan_kernel(unsigned* ghist, 
   }
 }
 
+constexpr int kRowBlock = 256;
+static_assert(kRowBlock == kSortBlock, "the scatter kernel scans the digit totals with the row scan's helper");
+
+// Block-wide exclusive scan of one value per thread (kRowBlock threads);
+// *total receives the sum.  Every thread of the block must call it.
+__device__ inline unsigned block_exclusive_scan(unsigned v, unsigned* total) {
+  __shared__ unsigned s_w[kRowBlock / 64];
+  const int lane = __lane_id(), wave = static_cast<int>(threadIdx.x) >> 6;
+  unsigned incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned x = __shfl_up(incl, o);
+    if (lane >= o) incl += x;
+  }
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  unsigned before = 0, sum = 0;
+  for (int w = 0; w < kRowBlock / 64; ++w) {
+    const unsigned t = s_w[w];
+    if (w < wave) before += t;
+    sum += t;
+  }
+  __syncthreads();  // s_w is written again by the next call
+  *total = sum;
+  return before + incl - v;
+}
+
+// Exclusive scan of each digit's row of per-tile counts, ghist[d * ntiles
+// + t], in place, one block per digit; the row's total goes to
+// ghist[256 * ntiles + d].  The scatter kernel adds the digits' bases.
+__global__ void __launch_bounds__(kRowBlock) sort_rowscan_kernel(unsigned* ghist, uint64_t ntiles) {
+  unsigned* row = ghist + static_cast<uint64_t>(blockIdx.x) * ntiles;
+  unsigned carry = 0;
+  for (uint64_t c = 0; c < ntiles; c += static_cast<uint64_t>(kRowBlock) * 4) {
+    const uint64_t i0 = c + static_cast<uint64_t>(threadIdx.x) * 4;
+    unsigned v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = i0 + q < ntiles ? row[i0 + q] : 0u;
+    unsigned tot = 0;
+    unsigned run = carry + block_exclusive_scan(v[0] + v[1] + v[2] + v[3], &tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (i0 + q < ntiles) row[i0 + q] = run;
+      run += v[q];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) ghist[256 * ntiles + blockIdx.x] = carry;
+}
+
+// goff: the row-scanned counts and, after them, the 256 digit totals.
 __global__ void __launch_bounds__(kSortBlock) sort_scatter_kernel(const uint64_t* kin, const uint32_t* vin,
                                                                   uint64_t* kout, uint32_t* vout,
                                                                   uint64_t n, int shift,
@@ -91,7 +141,9 @@ __global__ void __launch_bounds__(kSortBlock) sort_scatter_kernel(const uint64_t
   const int wave = tid >> 6;
   const uint64_t t = blockIdx.x;
   run[tid] = 0;
-  base[tid] = goff[static_cast<uint64_t>(tid) * ntiles + t];
+  unsigned all = 0;
+  const unsigned digit_base = block_exclusive_scan(goff[256 * ntiles + static_cast<uint64_t>(tid)], &all);
+  base[tid] = digit_base + goff[static_cast<uint64_t>(tid) * ntiles + t];
   for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
   __syncthreads();
   for (int s = 0; s < kItems; ++s) {
@@ -207,6 +259,7 @@ int grid_for(uint64_t n, int block) {
 }  // namespace
 
 uint64_t sort_tiles(uint64_t n) { return (n + kTile - 1) / kTile; }
+uint64_t sort_ghist_len(uint64_t n) { return 256 * (std::max<uint64_t>(1, sort_tiles(n)) + 1); }
 
 void launch_exclusive_scan_u32(unsigned* p, uint64_t len, hipStream_t s) {
   if (len == 0) return;
@@ -222,7 +275,7 @@ int radix_sort_pairs(SortBufs& b, uint64_t n, const int* bytes, int nbytes, hipS
     const int shift = 8 * bytes[p];
     hipLaunchKernelGGL(sort_hist_kernel, dim3(static_cast<unsigned>(ntiles)), dim3(kSortBlock), 0, s, b.key[cur], n, shift, b.ghist, ntiles);
     CPG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, b.ghist, 256 * ntiles);
+    hipLaunchKernelGGL(sort_rowscan_kernel, dim3(256), dim3(kRowBlock), 0, s, b.ghist, ntiles);
     CPG_HIP(hipGetLastError());
     hipLaunchKernelGGL(sort_scatter_kernel, dim3(static_cast<unsigned>(ntiles)), dim3(kSortBlock), 0, s, 
         b.key[cur], b.val[cur], b.key[cur ^ 1], b.val[cur ^ 1], n, shift, b.ghist, ntiles);
