@@ -103,6 +103,8 @@ ABI = {
     "cpg_query": (ctypes.c_int, [_H, _I64, ctypes.POINTER(QueryOpts)]),
     "cpg_num_paths": (_I64, [_H]),
     "cpg_get_costs": (_I64, [_H, _I64, _PF]),
+    "cpg_host_alloc": (ctypes.c_void_p, [_I64]),
+    "cpg_host_free": (ctypes.c_int, [ctypes.c_void_p]),
     "cpg_get_nodes": (_I64, [_H, _I64, ctypes.POINTER(CpgNode)]),
     "cpg_get_path_vertices": (_I64, [_H, _I64, _PI, _I64]),
     "cpg_timings": (ctypes.c_int, [_H, ctypes.POINTER(Timing)]),
@@ -180,6 +182,37 @@ def lib():
 
 class CpgError(RuntimeError):
     pass
+
+
+class HostArray:
+    """A float32 numpy array in pinned host memory (cpg_host_alloc), for
+    result copies that go straight from the device.  The memory lives until
+    close(); the array must not be used after that."""
+
+    def __init__(self, n, library=None):
+        self.L = library or lib()
+        n = max(int(n), 1)
+        self.p = self.L.cpg_host_alloc(4 * n)
+        if not self.p:
+            raise CpgError(self.L.cpg_last_error().decode())
+        self.array = np.ctypeslib.as_array((ctypes.c_float * n).from_address(self.p))
+
+    def close(self):
+        if self.p:
+            self.array = None
+            self.L.cpg_host_free(self.p)
+            self.p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        self.close()
 
 
 def _check(rc, L=None):
@@ -341,8 +374,14 @@ class CpGen:
     def num_paths(self):
         return self.L.cpg_num_paths(self.h)
 
-    def get_slacks(self, k):
-        out = np.zeros(max(int(k), 1), np.float32)
+    def get_slacks(self, k, out=None):
+        """First k costs (ascending).  out: a float32 array of at least k
+        elements to fill (a HostArray's, pinned, avoids staging); the
+        returned array is a view of it."""
+        if out is None:
+            out = np.empty(max(int(k), 1), np.float32)
+        elif out.dtype != np.float32 or not out.flags.c_contiguous or len(out) < max(int(k), 1):
+            raise ValueError("out must be a contiguous float32 array of at least k elements")
         n = _c(self, self.L.cpg_get_costs(self.h, int(k), _fptr(out)))
         return out[:n]
 

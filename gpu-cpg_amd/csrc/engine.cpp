@@ -2957,6 +2957,22 @@ int cpg_server_get_stats(cpg_server* s, cpg_server_stats* out) {
 
 double cpg_now_ms(void) { return now_ms(); }
 
+void* cpg_host_alloc(int64_t bytes) {
+  return guarded<void*>(nullptr, [&]() -> void* {
+    if (bytes <= 0) throw std::runtime_error("cpg_host_alloc needs a positive size");
+    void* p = nullptr;
+    CPG_HIP(hipHostMalloc(&p, static_cast<size_t>(bytes), hipHostMallocDefault));
+    return p;
+  });
+}
+
+int cpg_host_free(void* p) {
+  return guarded<int>(-1, [&]() {
+    if (p) CPG_HIP(hipHostFree(p));
+    return 0;
+  });
+}
+
 int cpg_cache_enable(cpg_handle* h, int enabled) {
   CPG_CHECK_HANDLE(h, -1)
   h->cache_enabled = enabled != 0;

@@ -304,6 +304,13 @@ int cpg_server_release(cpg_server* s, int64_t id);
 int cpg_server_get_stats(cpg_server* s, cpg_server_stats* out);
 /* The engine's steady clock in ms (for arrival schedules). */
 double cpg_now_ms(void);
+/* Pinned (page-locked) host memory for result buffers: cpg_get_costs and
+ * the like copy into it straight from the device, without the staging a
+ * pageable buffer needs.  No counterpart in the reference (its get_slacks
+ * fills a std::vector).  cpg_host_alloc returns null on failure (see
+ * cpg_last_error); cpg_host_free takes what it returned. */
+void* cpg_host_alloc(int64_t bytes);
+int cpg_host_free(void* p);
 
 /* ---- static cache, gpucpg/gpucpg.cuh:186-189 ---- */
 int cpg_cache_enable(cpg_handle* h, int enabled);

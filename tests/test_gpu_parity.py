@@ -476,3 +476,21 @@ def test_mem_budget_bytes_bounds_the_node_store():
         g.report_paths(50000, max_nodes=60000, mem_budget_bytes=sfxt_bytes + 61 * 61000)
     g.report_paths(50000, max_nodes=60000)
     assert g.timings()["retries"] >= 1
+
+
+def test_costs_into_pinned_host_array():
+    """get_slacks into a pinned host array (cpg_host_alloc) gives the same
+    costs as into an ordinary one, for a prefix and for all K; an array
+    shorter than K is refused."""
+    n, u, v, w = cpgamd.gen_preset(1)
+    g = engine_for(n, u, v, w)
+    say("case costs into pinned host memory")
+    g.report_paths(20000)
+    want = g.get_slacks(20000)
+    with cpgamd.HostArray(20000) as host:
+        got = g.get_slacks(20000, out=host.array)
+        np.testing.assert_array_equal(got, want)
+        got = g.get_slacks(5000, out=host.array)
+        np.testing.assert_array_equal(got, want[:5000])
+        with pytest.raises(ValueError):
+            g.get_slacks(20001, out=host.array)
