@@ -1316,18 +1316,35 @@ __global__ void __launch_bounds__(kSfxtBlock) sfxt_levels_parts_kernel(SfxtParts
   levels_multi_part(P.a, P.m, blockIdx.x - static_cast<unsigned>(P.b0), static_cast<unsigned>(P.nb));
 }
 
+// Logical index of a block such that the blocks the hardware places on one
+// XCD (round-robin: block b on XCD b % 8) take consecutive indices, so that
+// a kernel giving each block one contiguous chunk gives each XCD one
+// contiguous range (identity when the grid is not a multiple of 8).
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
+  constexpr unsigned kX = 8;
+  return nb % kX == 0 ? (b % kX) * (nb / kX) + b / kX : b;
+}
+
 // Deviation records of every settled vertex in fanout order, their smallest
 // delta, and the vertex record without its nsv pointer.  Lists of up to
 // kThreadSort records are sorted here (stable insertion sort by delta);
 // longer ones are queued for sfxt_sort_mid_kernel (up to kWaveSort, from
 // the front of big[]) or sfxt_sort_big_kernel (from the back).  The loop
 // is wave-uniform so the queue reservations are wave-aggregated.
+// Each block takes one contiguous chunk of vertices and each XCD one range
+// of chunks (xcd_block): a vertex's fanouts sit on the levels just below
+// it, and with level-ordered ids their dist/distf words then lie in the
+// range the same XCD's L2 already holds.
 __global__ void __launch_bounds__(256) sfxt_devs_kernel(SfxtArgs a) {
-  for (int base = blockIdx.x * blockDim.x; base < a.n; base += gridDim.x * blockDim.x) {
+  const int chunk = static_cast<int>(((static_cast<int64_t>(a.n) + gridDim.x - 1) / gridDim.x + 255) / 256 * 256);
+  const int64_t lo64 = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * chunk;
+  const int lo = static_cast<int>(lo64 < a.n ? lo64 : a.n);
+  const int hi = static_cast<int>(lo64 + chunk < a.n ? lo64 + chunk : a.n);
+  for (int base = lo; base < hi; base += blockDim.x) {
     const int u = base + threadIdx.x;
     int cnt = 0;
     bool sorted = true;
-    if (u < a.n) {
+    if (u < hi) {
       const int b = a.oadjp[u], e = a.oadjp[u + 1];
       if (a.level[u] < 0 || a.dist[u] == kIntMax) {  // unsettled or reaches no chosen sink
         a.vrec[u] = default_vrec();
