@@ -54,15 +54,18 @@ struct ChainWalk {
     }
     *from = v;
     if (mode == 0) {
+      // the vertex record and its deviation range are loaded together (one
+      // round trip where the walk stops here, not two); j and je are set
+      // either way, and only read once the walk scans this vertex
       const int4 r = vrec[v];
+      const int2 di = devinfo[v];
       ++*hops;
+      j = di.x;
+      je = di.x + di.y;
       if (__int_as_float(r.x) > X) {
         v = r.y;
         if (v == -1) mode = 2;
       } else {
-        const int2 di = devinfo[v];
-        j = di.x;
-        je = di.x + di.y;
         sv = r.z;
         dv = __int_as_float(r.w);
         mode = 1;
