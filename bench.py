@@ -33,6 +33,8 @@ Prints one JSON line.  Besides the contract fields it carries
                   edges), 16 dense frontiers: wave64 bitmap path vs int8 MFMA
                   tile path, kernel ms (HIP events), tiles, achieved int8
                   TOPS against the 5.0 POPS dense peak, both paths equal
+  ingest          the 1.5M-vertex DAG loaded from the text format (parsed on
+                  all host threads) and from the binary fanout CSR: sizes, ms
   served          8 resident graphs (200K-2.6M vertices, up to 70M edges),
                   a seeded stream of 500 queries each with an endpoint
                   subset (1-100% of sinks) and K log-uniform 1K-1M:
@@ -198,6 +200,38 @@ def bvss_section(np, cpgamd, nfront=16, span=4096, reps=3):
     return out
 
 
+def ingest_section(cpgamd, n, u, v, w):
+    """Loading the 1.5M-vertex DAG from a file: the reference's text format
+    (one edge per line, parsed on all host threads) and the binary fanout
+    CSR; each load builds the host CSR the engine uploads."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="cpg_ingest_")
+    try:
+        tp, bp = os.path.join(d, "g.txt"), os.path.join(d, "g.bin")
+        cpgamd.write_text(n, u, v, w, tp)
+        g = cpgamd.CpGen()
+        s0 = time.perf_counter()
+        g.read_input(tp)
+        text_ms = (time.perf_counter() - s0) * 1e3
+        g.write_to_csr_bin(bp)
+        g2 = cpgamd.CpGen()
+        s0 = time.perf_counter()
+        g2.read_input_bin(bp)
+        bin_ms = (time.perf_counter() - s0) * 1e3
+        same = g2.num_verts() == g.num_verts() == n and g2.num_edges() == g.num_edges() == len(u)
+        out = {"text_mb": round(os.path.getsize(tp) / 1e6, 1), "text_ms": round(text_ms, 1),
+               "bin_mb": round(os.path.getsize(bp) / 1e6, 1), "bin_ms": round(bin_ms, 1),
+               "same_graph": bool(same), "parser_threads": os.cpu_count()}
+        g.close()
+        g2.close()
+        log("ingest: text %.1f MB in %.1f ms, binary CSR %.1f MB in %.1f ms"
+            % (out["text_mb"], text_ms, out["bin_mb"], bin_ms))
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def served_section(np, cpgamd, nq=500, nsat=96, loads=((0.8, 500), (1.5, 300))):
     """Served stream of endpoint-subset queries on 8 resident graphs: the
     first nsat queries saturated (served engine vs plain queries one at a
@@ -235,8 +269,12 @@ def served_section(np, cpgamd, nq=500, nsat=96, loads=((0.8, 500), (1.5, 300))):
     plain_ms = (time.perf_counter() - s0) * 1e3
     log("served: plain saturated %.1f ms (%d queries)" % (plain_ms, nsat))
     S = cpgamd.Server()
+    s0 = time.perf_counter()
     for n, u, v, w in graphs:
         S.add_graph(n, u, v, w)
+    load_ms = (time.perf_counter() - s0) * 1e3
+    log("served: 8 graphs (%d vertices, %d edges) added in %.1f ms"
+        % (sum(x[0] for x in graphs), sum(len(x[1]) for x in graphs), load_ms))
 
     def drain(batch):
         """Runs the queued queries to completion and copies their costs out
@@ -331,7 +369,7 @@ def served_section(np, cpgamd, nq=500, nsat=96, loads=((0.8, 500), (1.5, 300))):
                           "served_queries_per_s": round(nsat / (served_ms * 1e-3), 2),
                           "plain_queries_per_s": round(nsat / (plain_ms * 1e-3), 2),
                           "speedup": round(plain_ms / served_ms, 3), "served_split": split},
-            "poisson": poisson,
+            "poisson": poisson, "graph_load_ms": round(load_ms, 1),
             "exact": "%d/%d" % (exact, nsat), "server": server_stats}
 
 
@@ -347,6 +385,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-bvss", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--served-queries", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -501,6 +540,8 @@ def main():
                             and pyoracle.first_bit_mismatch(cpu_costs, costs) == -1)
     host.close()
     g.close()  # every engine object is closed before exit, none from a finalizer
+    if not args.no_ingest:
+        out["ingest"] = ingest_section(cpgamd, n, u, v, w)
     if not args.no_batched:
         out["batched"] = batched_section(np, cpgamd)
     if not args.no_served:

@@ -210,7 +210,7 @@ void launch_round_log(const PfxtCtl* ctl, unsigned long long* rlog, int cap, hip
 
 // ---------------- multi-query PFXT (batched designs) ----------------
 constexpr int kMqBins = 1024;                 // per-query histogram bins
-constexpr int kMqHistStride = 3 * kMqBins;    // counts, per-bin max key, unexpanded counts
+constexpr int kMqHistStride = 2 * kMqBins;    // counts, then unexpanded counts
 
 struct MqQuery {
   uint32_t ukey;                  // bound (kKeyInf = none)

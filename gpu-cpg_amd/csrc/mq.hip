@@ -10,8 +10,12 @@
 //      its own kMqBins bins: Q x kMqBins bins keep contention low),
 //   2. the batch threshold from its unexpanded paths below U_q,
 //   3. a count pass over the batch histogramming its children (< U_q),
-//   4. U_q tightened to the largest key of the bin where the running count
-//      of its stored paths and counted children reaches K_q,
+//   4. U_q tightened to the top of the bin where the running count of its
+//      stored paths and counted children reaches K_q (the bin's upper
+//      edge: at least K_q paths cost no more; a per-bin largest key would
+//      be tighter by under one of the kMqBins bins, at the price of a
+//      second global atomic per counted child, and atomics are performed
+//      at memory),
 //   5. a fill pass storing its children <= U_q (inclusive: the counted
 //      children at U_q are needed).
 // The per-query semantics equal the single-query rounds' (see pfxt.hip);
@@ -102,8 +106,7 @@ __global__ void __launch_bounds__(kMqBlock) mq_roots_kernel(MqArgs a, const int*
 }
 
 // Histograms of every stored path at or below its query's bound: nonzero
-// keys with each bin's largest key, and the unexpanded paths below the
-// bound.
+// keys, and the unexpanded paths below the bound.
 __global__ void __launch_bounds__(kMqBlock) mq_hist_kernel(MqArgs a) {
   const uint64_t n = a.ctl->node_n < a.nodes.cap ? a.ctl->node_n : a.nodes.cap;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -114,11 +117,8 @@ __global__ void __launch_bounds__(kMqBlock) mq_hist_kernel(MqArgs a) {
     if (qs.done || k > qs.ukey) continue;
     const uint32_t b = mq_bin(k, qs.klo, qs.shift);
     unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
-    if (!key_is_zero(k)) {
-      atomicAdd(&h[b], 1u);
-      atomicMax(&h[kMqBins + b], k);
-    }
-    if (k < qs.ukey && a.nodes.flag[i] == 0) atomicAdd(&h[2 * kMqBins + b], 1u);
+    if (!key_is_zero(k)) atomicAdd(&h[b], 1u);
+    if (k < qs.ukey && a.nodes.flag[i] == 0) atomicAdd(&h[kMqBins + b], 1u);
   }
 }
 
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(kMqSelBlock) mq_select_batch_kernel(MqArgs a, 
   const unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
   unsigned long long total = 0;
   const bool take_all = all || qs->all;
-  const uint32_t b = mq_crossing(h + 2 * kMqBins, take_all ? ~0ull : qs->batch_target, &total);
+  const uint32_t b = mq_crossing(h + kMqBins, take_all ? ~0ull : qs->batch_target, &total);
   if (threadIdx.x == 0) {
     qs->all = 0;
     if (total == 0) {
@@ -201,11 +201,7 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
       if (!kFill) {
         if (emit) {
           const uint32_t k = float_key(cc);
-          if (!key_is_zero(k)) {
-            const uint32_t b = mq_bin(k, qs.klo, qs.shift);
-            atomicAdd(&h[b], 1u);
-            atomicMax(&h[kMqBins + b], k);
-          }
+          if (!key_is_zero(k)) atomicAdd(&h[mq_bin(k, qs.klo, qs.shift)], 1u);
         }
         continue;
       }
@@ -247,7 +243,7 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
   if (kFill && __lane_id() == 0 && nonmono) atomicAdd(&ctl->nonmono, nonmono);  // once per expansion
 }
 
-// One block per query: U_q = the largest key of the bin where the running
+// One block per query: U_q = the upper edge of the bin where the running
 // count of stored paths and counted children reaches K_q.
 __global__ void __launch_bounds__(kMqSelBlock) mq_select_bound_kernel(MqArgs a) {
   const int q = blockIdx.x;
@@ -259,8 +255,8 @@ __global__ void __launch_bounds__(kMqSelBlock) mq_select_bound_kernel(MqArgs a) 
   if (threadIdx.x == 0) {
     uint32_t uk = qs->ukey;
     if (b < static_cast<uint32_t>(kMqBins)) {
-      const uint32_t top = h[kMqBins + b];
-      if (top < uk) uk = top;
+      const uint64_t up = static_cast<uint64_t>(qs->klo) + ((static_cast<uint64_t>(b) + 1) << qs->shift) - 1;
+      if (up < uk) uk = static_cast<uint32_t>(up);
     }
     qs->ukey = uk;
     qs->shift = mq_shift(qs->klo, uk);
