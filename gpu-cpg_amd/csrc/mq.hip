@@ -168,13 +168,26 @@ __global__ void __launch_bounds__(kMqBlock) mq_make_batch_kernel(MqArgs a) {
 
 // Count pass (kFill false): histograms each query's children < U_q.
 // Fill pass: stores each query's children <= U_q.
+// In the count pass a block counts the children of the query its first
+// parent belongs to in an LDS histogram and adds its nonzero bins to the
+// query's global histogram at the end of the step: a block's 256
+// consecutive batch entries mostly belong to one query (a parent's
+// children are stored together), and every global atomic is performed at
+// memory.  Parents of other queries count with global atomics.
 template <bool kFill>
 __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
+  __shared__ unsigned s_h[kMqBins];
+  __shared__ int s_q;
   MqCtl* ctl = a.ctl;
   const uint64_t nb = ctl->batch_n;
   unsigned long long hops = 0, devv = 0, nonmono = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < nb;
-       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+       base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {  // block-uniform
+    if (!kFill) {
+      if (threadIdx.x == 0) s_q = a.nq[a.batch[base]];
+      for (int j = threadIdx.x; j < kMqBins; j += blockDim.x) s_h[j] = 0u;
+      __syncthreads();
+    }
     const uint64_t i = base + threadIdx.x;
     int p = 0, lvl = 0, q = 0, to = -1;
     float c = 0.0f;
@@ -192,6 +205,7 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
     ChainWalk w;
     w.start(i < nb, c, to, U, qs.ukey != kKeyInf, qs.mag);
     unsigned* h = a.hist + static_cast<size_t>(q) * kMqHistStride;
+    const bool local = !kFill && q == s_q;  // count in the block's LDS histogram
     unsigned stored = 0;  // fill: children this lane stored for its parent
     while (__ballot(w.mode != 2)) {
       float cc = 0.0f;
@@ -201,7 +215,11 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
       if (!kFill) {
         if (emit) {
           const uint32_t k = float_key(cc);
-          if (!key_is_zero(k)) atomicAdd(&h[mq_bin(k, qs.klo, qs.shift)], 1u);
+          if (!key_is_zero(k)) {
+            const uint32_t b = mq_bin(k, qs.klo, qs.shift);
+            if (local) atomicAdd(&s_h[b], 1u);
+            else atomicAdd(&h[b], 1u);
+          }
         }
         continue;
       }
@@ -235,6 +253,15 @@ __global__ void __launch_bounds__(kMqBlock) mq_walk_kernel(MqArgs a) {
         if (__lane_id() == leader) atomicAdd(&a.qs[lq].stored, sum);
         pend &= ~__ballot(mine);
       }
+    }
+    if (!kFill) {  // the block's LDS counts into its query's histogram
+      __syncthreads();
+      unsigned* hq = a.hist + static_cast<size_t>(s_q) * kMqHistStride;
+      for (int j = threadIdx.x; j < kMqBins; j += blockDim.x) {
+        const unsigned v = s_h[j];
+        if (v) atomicAdd(&hq[j], v);
+      }
+      __syncthreads();  // s_h and s_q are set again for the next step
     }
   }
   (void)hops;
