@@ -1853,6 +1853,10 @@ struct cpg_server {
         {
           std::lock_guard<std::mutex> lk(stats_mu);
           stats.probe_ms += now_ms() - tp;
+          if (trace_on())
+            std::fprintf(stderr, "served t %.2f probe query %lld graph %d k %lld trials %d ms %.2f\n", now_ms(),
+                         static_cast<long long>(q.id), q.graph, static_cast<long long>(q.k), h.timing.probes,
+                         now_ms() - tp);
         }
         PfxtCtl pc = h.read_pctl();
         if (pc.err) throw std::runtime_error("PFXT walk exceeded its step bound");
@@ -2008,6 +2012,10 @@ struct cpg_server {
         total += parts.p[i].nb;
       }
       launch_sfxt_levels_parts(parts, total, sstream);
+      if (trace_on())
+        for (size_t li = 0; li < list.size(); ++li)
+          std::fprintf(stderr, "served t %.2f start graph %d queries %zu blocks %d\n", now_ms(), list[li].first,
+                       list[li].second.size(), part_of[li] >= 0 ? parts.p[part_of[li]].nb : 0);
     }
     if (!ev_levels) CPG_HIP(hipEventCreateWithFlags(&ev_levels, hipEventDisableTiming));
     CPG_HIP(hipEventRecord(ev_levels, sstream));
@@ -2057,6 +2065,8 @@ struct cpg_server {
   // lock one at a time.
   void finish_groups(const std::vector<int>& gis) {
     if (gis.empty()) return;
+    if (trace_on())
+      for (int gi : gis) std::fprintf(stderr, "served t %.2f ready graph %d\n", now_ms(), gi);
     std::vector<std::vector<unsigned>> maxd(gis.size());
     std::vector<std::vector<Query*>> qp(gis.size());  // looked up here, not in the threads
     // every group is checked before any is committed: a failed check puts
@@ -2334,8 +2344,8 @@ struct cpg_server {
         Gq.nodes_per_k = 0.5 * Gq.nodes_per_k + 0.5 * std::max(1.0, r);
         dead_nodes += static_cast<uint64_t>(used);
         if (trace_on())
-          std::fprintf(stderr, "served query %lld graph %d k %lld stored %llu seed %llu roots %zu ratio %.1f\n",
-                       static_cast<long long>(q.id), q.graph, static_cast<long long>(q.k),
+          std::fprintf(stderr, "served t %.2f query %lld graph %d k %lld stored %llu seed %llu roots %zu ratio %.1f\n",
+                       now_ms(), static_cast<long long>(q.id), q.graph, static_cast<long long>(q.k),
                        static_cast<unsigned long long>(hq[sl].stored),
                        static_cast<unsigned long long>(q.seed_n), q.roots.size(), r);
       }

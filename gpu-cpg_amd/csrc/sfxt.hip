@@ -1406,10 +1406,23 @@ __device__ inline uint32_t delta_rank_key(int bits) {
 // Deviation records with one wave per vertex, for graphs whose fanouts
 // are long on average: lanes take the fanout edges 64 at a time and a
 // ballot compacts the deviation records in fanout order.  A fanout of at
-// most 64 edges keeps its records in registers, ranks them there (ties in
-// fanout order, as sfxt_sort_mid_kernel) and stores each once, sorted;
-// longer fanouts store their records in fanout order and are queued for
+// most 128 edges keeps its records in registers (two per lane), ranks them
+// there (ties in fanout order, as sfxt_sort_mid_kernel) and stores each
+// once, sorted: each valid record's key is broadcast once and compared
+// with both of a lane's keys, so ranking costs one shuffle per record.
+// Longer fanouts store their records in fanout order and are queued for
 // sfxt_sort_big_kernel.  vrec and devinfo as in sfxt_devs_kernel.
+__device__ inline bool devs_record(const SfxtArgs& a, int j, int s, float df, int4& rec) {
+  const int w = a.oadj[j];
+  if (w == s || a.dist[w] == kIntMax) return false;
+  const float fw = a.distf[w];
+  const float we = a.owgt[j];
+  const float delta = static_cast<float>(static_cast<double>(fw) + static_cast<double>(we) -
+                                         static_cast<double>(df));
+  rec = make_int4(w, __float_as_int(we), __float_as_int(fw), __float_as_int(delta));
+  return true;
+}
+
 __global__ void __launch_bounds__(256) sfxt_devs_wave_kernel(SfxtArgs a) {
   const int lane = __lane_id();
   const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -1427,45 +1440,49 @@ __global__ void __launch_bounds__(256) sfxt_devs_wave_kernel(SfxtArgs a) {
     const float df = a.distf[u];
     int cnt = 0;
     float mind = __builtin_huge_valf();
-    int4 rec = make_int4(0, 0, 0, 0);
-    bool ok = false;
-    int pos = 0;
-    for (int j0 = b; j0 < e; j0 += 64) {
-      const int j = j0 + lane;
-      ok = false;
-      if (j < e) {
-        const int w = a.oadj[j];
-        if (w != s && a.dist[w] != kIntMax) {
-          const float fw = a.distf[w];
-          const float we = a.owgt[j];
-          const float delta = static_cast<float>(static_cast<double>(fw) + static_cast<double>(we) -
-                                                 static_cast<double>(df));
-          rec = make_int4(w, __float_as_int(we), __float_as_int(fw), __float_as_int(delta));
-          mind = fminf(mind, delta);
-          ok = true;
-        }
+    if (e - b <= 128) {
+      int4 r0 = make_int4(0, 0, 0, 0), r1 = make_int4(0, 0, 0, 0);
+      const bool ok0 = b + lane < e && devs_record(a, b + lane, s, df, r0);
+      const bool ok1 = b + 64 + lane < e && devs_record(a, b + 64 + lane, s, df, r1);
+      if (ok0) mind = fminf(mind, __int_as_float(r0.w));
+      if (ok1) mind = fminf(mind, __int_as_float(r1.w));
+      const uint64_t m0 = __ballot(ok0), m1 = __ballot(ok1);
+      cnt = static_cast<int>(__popcll(m0) + __popcll(m1));
+      const uint32_t k0 = ok0 ? delta_rank_key(r0.w) : 0u;
+      const uint32_t k1 = ok1 ? delta_rank_key(r1.w) : 0u;
+      // rank = records ordering before this one: a smaller key, or an
+      // equal key earlier in fanout order (every first-half record is
+      // earlier than every second-half one)
+      int rank0 = 0, rank1 = 0;
+      for (uint64_t mm = m0; mm; mm &= mm - 1) {
+        const int jj = static_cast<int>(__builtin_ctzll(mm));
+        const uint32_t kj = __shfl(k0, jj);
+        rank0 += (kj < k0 || (kj == k0 && jj < lane)) ? 1 : 0;
+        rank1 += kj <= k1 ? 1 : 0;
       }
-      const uint64_t bm = __ballot(ok);
-      pos = cnt + static_cast<int>(__popcll(bm & lanemask_lt()));
-      if (e - b > 64 && ok) a.devs[b + pos] = rec;  // long fanout: fanout order, sorted later
-      cnt += static_cast<int>(__popcll(bm));
+      for (uint64_t mm = m1; mm; mm &= mm - 1) {
+        const int jj = static_cast<int>(__builtin_ctzll(mm));
+        const uint32_t kj = __shfl(k1, jj);
+        rank0 += kj < k0 ? 1 : 0;
+        rank1 += (kj < k1 || (kj == k1 && jj < lane)) ? 1 : 0;
+      }
+      if (ok0) a.devs[b + rank0] = r0;
+      if (ok1) a.devs[b + rank1] = r1;
+    } else {
+      for (int j0 = b; j0 < e; j0 += 64) {
+        int4 rec = make_int4(0, 0, 0, 0);
+        const bool ok = j0 + lane < e && devs_record(a, j0 + lane, s, df, rec);
+        if (ok) mind = fminf(mind, __int_as_float(rec.w));
+        const uint64_t bm = __ballot(ok);
+        if (ok) a.devs[b + cnt + static_cast<int>(__popcll(bm & lanemask_lt()))] = rec;  // sorted later
+        cnt += static_cast<int>(__popcll(bm));
+      }
+      if (lane == 0) {
+        const unsigned long long sl = atomicAdd(&a.ctl->nbig_l, 1ull);
+        a.big[a.n - 1 - static_cast<int>(sl)] = u;
+      }
     }
     for (int o = 32; o > 0; o >>= 1) mind = fminf(mind, __shfl_xor(mind, o));
-    if (e - b <= 64) {
-      // rank among the wave's records: smaller key first, ties by position
-      const uint32_t key = ok ? delta_rank_key(rec.w) : 0xffffffffu;
-      const int mypos = ok ? pos : 64;
-      int rank = 0;
-      for (int jj = 0; jj < 64; ++jj) {
-        const uint32_t kj = __shfl(key, jj);
-        const int pj = __shfl(mypos, jj);
-        rank += (kj < key || (kj == key && pj < mypos)) ? 1 : 0;
-      }
-      if (ok) a.devs[b + rank] = rec;
-    } else if (lane == 0) {
-      const unsigned long long sl = atomicAdd(&a.ctl->nbig_l, 1ull);
-      a.big[a.n - 1 - static_cast<int>(sl)] = u;
-    }
     if (lane == 0) {
       a.vrec[u] = make_int4(__float_as_int(mind), -1, s, __float_as_int(df));
       a.devinfo[u] = make_int2(b, cnt);

@@ -162,3 +162,39 @@ def test_emulated_lds_queue_spill(emu, method):
         assert bits_equal(want, g.get_slacks(50))
     finally:
         g.close()
+
+
+def test_emulated_dense_fanouts_equal_oracle(emu):
+    """A graph of mean fanout far above 8, so the deviation index is built
+    one wave per vertex: fanouts of up to 64 records, of 65-128 (two per
+    lane, ranked in registers) and of 129-200 (queued for the block sort),
+    with delays from three values so that many deviation records tie.  The
+    query's costs match the oracle's bit for bit."""
+    rng = np.random.default_rng(23)
+    width, levels = 200, 10
+    n = width * levels
+    u, v = [], []
+    for x in range(n - width):
+        lvl = x // width
+        hi = min(n, (lvl + 4) * width)
+        pool = np.arange((lvl + 1) * width, hi)
+        band = rng.random()
+        deg = int(rng.integers(1, 11) if band < 0.4 else rng.integers(65, 129) if band < 0.8
+                  else rng.integers(129, 201))
+        tgt = rng.choice(pool, min(deg, len(pool)), replace=False)
+        u += [x] * len(tgt)
+        v += tgt.tolist()
+    u = np.array(u, np.int32)
+    v = np.array(v, np.int32)
+    assert len(u) >= 8 * n
+    w = rng.choice(np.array([0.125, 0.25, 0.5], np.float32), len(u))
+    og = pyoracle.OracleGraph.from_edges(n, u, v, w)
+    dist, succ, level, L = og.sfxt()
+    g = emu.CpGen()
+    try:
+        g.load_edges(n, u, v, w)
+        g.report_paths(3000)
+        want, _ = og.topk(dist, succ, 3000)
+        assert bits_equal(want, g.get_slacks(3000))
+    finally:
+        g.close()
