@@ -1595,7 +1595,9 @@ struct cpg_server {
     std::vector<SfxtArgs> args;   // per query: its distance columns and index
     SfxtArgs base{};              // the group's shared level-loop arguments
     int lv = 0;                   // lift depth
-    std::vector<uint8_t> hmask;   // host copy of the subsets' sink flags
+    PinnedBuf<uint8_t> hmask;     // host copy of the subsets' sink flags (pinned: the copy is asynchronous)
+    PinnedBuf<SfxtCtl> hctl;      // the group's SFXT controls, copied out before its event: graph's, then each query's
+    PinnedBuf<unsigned> hmaxd;    // each query's largest finite |distf| (float bits), likewise
     hipEvent_t ev = nullptr;
     ~Graph() {
       if (ev) (void)hipEventDestroy(ev);
@@ -1947,13 +1949,14 @@ struct cpg_server {
       const size_t ns = h.g.sinks.size();
       M.masks.reserve(std::max<size_t>(1, ns * Q)); M.maxd.reserve(Q); M.ctls.reserve(Q);
       // one flag per sink (in sink order) and subset; padding subsets choose none
-      G.hmask.assign(ns * Q, 0);  // kept until the group finishes (asynchronous copy)
+      G.hmask.reserve(std::max<size_t>(1, ns * Q));  // kept until the group finishes (asynchronous copy)
+      std::memset(G.hmask.p, 0, ns * Q);
       for (size_t q = 0; q < ids.size(); ++q) {
         const Query& qq = *queries[ids[q]];
         for (size_t i = 0; i < ns; ++i)
-          G.hmask[q * ns + i] = qq.mask.empty() ? 1 : (qq.mask[h.g.sinks[i]] ? 1 : 0);
+          G.hmask.p[q * ns + i] = qq.mask.empty() ? 1 : (qq.mask[h.g.sinks[i]] ? 1 : 0);
       }
-      h2d(M.masks.p, G.hmask.data(), ns * Q, sstream);
+      h2d(M.masks.p, G.hmask.p, ns * Q, sstream);
       CPG_HIP(hipMemsetAsync(h.sctl.p, 0, sizeof(SfxtCtl), sstream));
       CPG_HIP(hipMemsetAsync(M.ctls.p, 0, sizeof(SfxtCtl) * Q, sstream));
       SfxtArgs a{};
@@ -2017,6 +2020,7 @@ struct cpg_server {
           std::fprintf(stderr, "served t %.2f start graph %d queries %zu blocks %d\n", now_ms(), list[li].first,
                        list[li].second.size(), part_of[li] >= 0 ? parts.p[part_of[li]].nb : 0);
     }
+    const double t_mid = now_ms();
     if (!ev_levels) CPG_HIP(hipEventCreateWithFlags(&ev_levels, hipEventDisableTiming));
     CPG_HIP(hipEventRecord(ev_levels, sstream));
     for (size_t li = 0; li < list.size(); ++li) {
@@ -2047,11 +2051,25 @@ struct cpg_server {
         qq.index = std::move(ix);
         G.args.push_back(aq);
       }
+      // the values check_group reads, copied out ahead of the group's event
+      // (so that finishing the group waits for no copy queued behind other
+      // streams' work)
+      const size_t qa = G.args.size();
+      G.hctl.reserve(qa + 1);
+      G.hmaxd.reserve(std::max<size_t>(qa, 1));
+      if (n > 0) {
+        d2h(G.hctl.p, h.sctl.p, 1, h.stream);
+        d2h(G.hctl.p + 1, G.mm.ctls.p, qa, h.stream);
+        d2h(G.hmaxd.p, G.mm.maxd.p, qa, h.stream);
+      }
       if (!G.ev) CPG_HIP(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
       CPG_HIP(hipEventRecord(G.ev, h.stream));
       G.busy = true;
       stats.sfxt_groups++;
     }
+    if (trace_on())
+      std::fprintf(stderr, "served t %.2f groups started: %zu graphs, loops %.2f ms, indexes %.2f ms\n", now_ms(),
+                   list.size(), t_mid - t0, now_ms() - t_mid);
     stats.sfxt_ms += now_ms() - t0;
   }
 
@@ -2132,12 +2150,13 @@ struct cpg_server {
     SfxtCtl c{};
     std::vector<SfxtCtl> cq(std::max(Q, 1));
     std::vector<unsigned> maxd(std::max(Q, 1), 0u);
-    if (h.g.n > 0) {
-      d2h(&c, h.sctl.p, 1, h.stream);
-      d2h(cq.data(), G.mm.ctls.p, Q, h.stream);
-      d2h(maxd.data(), G.mm.maxd.p, Q, h.stream);
+    if (h.g.n > 0) {  // copied out before the group's event, which has fired
+      c = G.hctl.p[0];
+      for (int q = 0; q < Q; ++q) {
+        cq[q] = G.hctl.p[1 + q];
+        maxd[q] = G.hmaxd.p[q];
+      }
     }
-    CPG_HIP(hipStreamSynchronize(h.stream));
     if (h.g.n > 0) {
       if (c.err) throw std::runtime_error("SFXT grid barrier did not complete");
       if (c.num_levels != h.diameter) throw std::runtime_error("SFXT levels differ from the graph's");
@@ -2212,7 +2231,7 @@ struct cpg_server {
   // it and the next queued queries on the same graph (at most 16), while
   // slots are free and the in-flight and starting indexes stay within the
   // budget (a lone query is taken whatever its size); the groups formed
-  // start together (one level-loop launch).  With nothing in flight,
+  // start in at most two level-loop launches (short loops first).  With nothing in flight,
   // admission waits for the first group in progress.
   void admit() {
     finish_groups(ready_groups());
@@ -2263,7 +2282,19 @@ struct cpg_server {
       in_batch[gi] = 1;
       batch.emplace_back(gi, std::move(ids));
     }
-    if (!batch.empty()) start_groups(batch);
+    if (!batch.empty()) {
+      // A launch lasts as long as its longest level loop, so graphs of at
+      // most half the batch's largest level count start in a launch of
+      // their own first: their queries reach the rounds without waiting
+      // for the long loops, which follow on the same stream.
+      int dmax = 0;
+      for (const auto& b : batch) dmax = std::max(dmax, static_cast<int>(graphs[b.first]->h->diameter));
+      std::vector<std::pair<int, std::vector<int64_t>>> shorter, longer;
+      for (auto& b : batch)
+        (2 * static_cast<int>(graphs[b.first]->h->diameter) <= dmax ? shorter : longer).push_back(std::move(b));
+      if (!shorter.empty()) start_groups(shorter);
+      start_groups(longer);
+    }
     if (in_flight == 0 && any_busy()) {
       // nothing to run meanwhile: wait for whichever groups finish first
       // (the graphs' index builds end at different times)
