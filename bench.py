@@ -50,6 +50,14 @@ import statistics
 import sys
 import time
 
+# The served section keeps eight graphs' streams, the level-loop stream and
+# the round stream busy at once.  With HIP's default of 4 hardware queues
+# per process those streams share queues, and work on a shared queue waits
+# behind another stream's long level loop; the process asks for 16 before
+# the HIP runtime starts (unless already given more).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or "0") < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-cpg_amd"))
 
@@ -370,7 +378,8 @@ def served_section(np, cpgamd, nq=500, nsat=96, loads=((0.8, 500), (1.5, 300))):
                           "plain_queries_per_s": round(nsat / (plain_ms * 1e-3), 2),
                           "speedup": round(plain_ms / served_ms, 3), "served_split": split},
             "poisson": poisson, "graph_load_ms": round(load_ms, 1),
-            "exact": "%d/%d" % (exact, nsat), "server": server_stats}
+            "exact": "%d/%d" % (exact, nsat), "server": server_stats,
+            "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")}
 
 
 def main():
