@@ -541,6 +541,21 @@ struct cpg_handle {
       // against 8.67 on one, the 256-design union graph 25 against 38 ms
       const int64_t records_per_level = g.m / std::max(1, last_diameter);
       a.place_stride = method == 5 && records_per_level <= kOneXcdRecords ? kXcds : 1;
+      // Queue entries per wave in that loop, on one XCD: as many as fill
+      // about one lane step of fanin records (64 x 3 records per step below
+      // a mean in-degree of 4, 64 x 8 above), at least 8, so that a level of
+      // few vertices with long in-degrees spreads over more waves.  The
+      // 2.6M-vertex graph of mean in-degree 27 (1,609 levels) takes 35.4 ms
+      // at 16 entries against 55.0 at 64, the 385K-vertex one 7.1 against
+      // 20.2; over all XCDs the 30M-edge DAG ran slower (11.2 against 8.4),
+      // so there every wave keeps 64.
+      int we = 64;
+      if (a.place_stride > 1) {
+        const double mean_in = static_cast<double>(g.m) / static_cast<double>(std::max(1, g.n));
+        const double per_step = 64.0 * (g.m < 4 * static_cast<int64_t>(g.n) ? 3 : 8);
+        while (we > 8 && we * mean_in > per_step) we >>= 1;
+      }
+      a.wave_entries = we;
       const bool dataflow = method == 2;
       if (dataflow && relabelled)
         throw std::runtime_error("the dataflow SFXT loop does not run on a relabelled graph");

@@ -61,6 +61,7 @@ struct SfxtArgs {
   // (the XCD count) every working block lands on one XCD under round-robin
   // dispatch
   int place_stride;
+  int wave_entries;  // edge-parallel loop: queue entries a wave takes at a time (1-64; 0 = 64)
   int64_t m;  // edges (the fence-free loop picks its records per step by m / n)
 };
 

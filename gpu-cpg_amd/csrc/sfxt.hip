@@ -445,8 +445,9 @@ __device__ __forceinline__ int queue_entry(const SfxtArgs& a, unsigned i) {
 // as levels_loop<false>, with the fanin release spread over the records of
 // a wave's vertices instead of over one vertex per lane.
 //
-// A wave takes 64 queue entries at a time (one per lane), settles them as
-// above, and prefix-sums their in-degrees over the wave; the wave's records
+// A wave takes 64 queue entries at a time (one per lane; fewer, by
+// wave_entries, where in-degrees are long), settles them as above, and
+// prefix-sums their in-degrees over the wave; the wave's records
 // then form one flat list, handed out kS per lane step (lane l takes list
 // positions r0 + s*64 + l), and a lane finds the vertex owning its position
 // by a binary search over the exclusive prefix sums held in the lanes
@@ -469,6 +470,9 @@ __device__ void levels_edges_loop(const SfxtArgs& a) {
   const int lane = __lane_id();
   const unsigned wave = static_cast<unsigned>(gtid) >> 6;
   const unsigned nwaves = nblocks * blockDim.x / 64u;
+  // entries per wave: fewer than 64 where in-degrees are long, so that a
+  // level's records spread over more waves (lanes past them hold none)
+  const unsigned we = a.wave_entries > 0 && a.wave_entries < 64 ? static_cast<unsigned>(a.wave_entries) : 64u;
   if (threadIdx.x == 0) s_qn = 0;
   __syncthreads();
 
@@ -484,11 +488,11 @@ __device__ void levels_edges_loop(const SfxtArgs& a) {
   float local_max = 0.0f;
   while (true) {
     if (gtid == 0) a.lvlp[d] = static_cast<int>(lb);
-    // wave-uniform loop over 64-entry chunks of the level
-    for (unsigned i0 = lb + wave * 64u; i0 < le; i0 += nwaves * 64u) {
+    // wave-uniform loop over we-entry chunks of the level
+    for (unsigned i0 = lb + wave * we; i0 < le; i0 += nwaves * we) {
       const unsigned i = i0 + static_cast<unsigned>(lane);
       int b = 0, deg = 0, du = kIntMax, tie = 0;
-      if (i < le) {
+      if (static_cast<unsigned>(lane) < we && i < le) {
         const int u = queue_entry(a, i);
         if (u >= 0) {
           const unsigned long long key = ld_dev64(a.best + u);
